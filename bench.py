@@ -1,0 +1,222 @@
+#!/usr/bin/env python3
+"""Benchmark of the L.A.Cosmic cleaning loop on one MI355X.
+
+Workload (default): the ragged batch -- 64 float64 frames with shapes drawn
+from {256x256, 512x2048, 1024x1024, 2048x4096, 4096x4096}, per-frame seeds and
+error maps, 2e-4 cosmic-ray tracks/px, settings (3, 5, 3), maxiter 4, mirror
+borders -- cleaned in one call of remove_cosmics_ragged_device with inputs
+resident in HBM. A step is one such call. Also reported:
+  * per_frame: the same 64 frames as 64 separate plain-path calls;
+  * fwd_bwd: 8 x 1024^2 noise-model frames, forward with donor recording then
+    the adjoint of loss = sum(w * cleaned), vs forward alone;
+  * roofline: the detection kernel's algorithmic bytes (8 B of clean image +
+    8 B of error map per processed pixel) over its event-timed duration;
+  * cpu_baseline: the oracle (the C restatement, one thread) on a bounded
+    sample of the same workload.
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "cleaned Mpix/s fp64 maxiter=4: ragged-batch vs per-frame; fwd+bwd Mpix/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def cpu_info():
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count()
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return cores, model
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--frames", type=int, default=64)
+    ap.add_argument("--no-per-frame", action="store_true")
+    ap.add_argument("--no-backward", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every GPU box in this setup has one GPU, and the "
+              "multi-GPU design (frames sharded by pixel count, no exchange) is described in "
+              "DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from cusmic_amd import _lib, engine, scenes
+
+    dev = torch.device("cuda", 0)
+    _lib.lib()
+    log("generating the ragged batch on the GPU")
+    batch = scenes.config3(dev, n=args.frames)
+    fr = engine.Frames()
+    for item in batch:
+        fr.add(item["data"], error=item["error"])
+    total = fr.total
+    mpix = total / 1e6
+    clean = torch.empty(total, dtype=torch.float64, device=dev)
+    mask = torch.empty(total, dtype=torch.uint8, device=dev)
+    log(f"{len(batch)} frames, {mpix:.1f} Mpx")
+
+    def step(profile=None):
+        return engine.run(fr, dev, 3.0, 5.0, 3.0, 4, _lib.MIRROR, clean=clean, mask=mask, profile=profile)
+
+    for i in range(args.warmup):
+        step()
+        log(f"warmup {i + 1}/{args.warmup}")
+    profiles = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        prof = {}
+        step(prof)
+        profiles.append(prof)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_step = 1e3 * elapsed / args.steps
+    value = mpix / (ms_step / 1e3)
+    log(f"ragged: {ms_step:.3f} ms/step, {value:.1f} Mpix/s")
+    flagged = int(mask.sum())
+    passes = step()[3]
+
+    detect_ms = float(np.mean([p["detect_ms"] for p in profiles]))
+    detect_px = float(np.mean([p["detect_pixels"] for p in profiles]))
+    launches = profiles[0]["detect_launches"]
+    bytes_per_px = 16.0  # clean image 8 B + error map 8 B per processed pixel
+    achieved = detect_px * bytes_per_px / (detect_ms / 1e3) / 1e9
+    per_launch_bytes = detect_px * bytes_per_px / launches
+    stage_ms = {k: float(np.mean([p[k] for p in profiles]))
+                for k in ("detect_ms", "replace_ms", "prologue_ms", "epilogue_ms")}
+    # algorithmic bytes of the whole call (B = 17 + M + I_d (8 + E) + I_r 8)
+    mean_passes = float(np.mean(passes))
+    call_bytes = total * (17 + 8 * 0 + mean_passes * 16 + max(0.0, mean_passes - 1) * 8) \
+        + 0.0
+    result = {
+        "metric": METRIC, "value": round(value, 2), "unit": "Mpix/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"ragged batch of {len(batch)} float64 frames (256^2..4096^2), "
+                               "per-frame error maps, 2e-4 CR tracks/px, contrast 3, thresholds 5/3, "
+                               "maxiter 4, mirror, one remove_cosmics_ragged_device call",
+                   "frames": len(batch), "megapixels": round(mpix, 3), "maxiter": 4,
+                   "flagged_pixels": flagged, "detection_passes_mean": mean_passes},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "detect", "algorithmic_bytes_per_launch": round(per_launch_bytes),
+                     "avg_launch_ms": round(detect_ms / launches, 4)},
+        "stages_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "call_roofline_frac": round(call_bytes / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+    }
+
+    if not args.no_per_frame:
+        log("per-frame plain path")
+        singles = []
+        for item in batch:
+            f1 = engine.Frames()
+            f1.add(item["data"], error=item["error"])
+            singles.append(f1)
+
+        def per_frame():
+            for f1 in singles:
+                engine.run(f1, dev, 3.0, 5.0, 3.0, 4, _lib.MIRROR)
+        per_frame()
+        torch.cuda.synchronize()
+        reps = max(1, args.steps // 4)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            per_frame()
+        torch.cuda.synchronize()
+        pf_ms = 1e3 * (time.perf_counter() - t0) / reps
+        result["per_frame"] = {"value": round(mpix / (pf_ms / 1e3), 2), "unit": "Mpix/s",
+                               "ms_per_batch": round(pf_ms, 3),
+                               "ragged_speedup": round(pf_ms / ms_step, 3)}
+        log(f"per-frame: {pf_ms:.3f} ms/batch")
+        del singles
+
+    if not args.no_backward:
+        log("forward+backward, 8 x 1024^2 noise-model frames")
+        from cusmic_amd import autograd
+
+        c4 = scenes.config4(dev)
+        data, w = c4["data"], c4["weights"]
+        mp4 = data.numel() / 1e6
+
+        def fwd():
+            x = data.clone().requires_grad_(True)
+            out, _ = autograd.remove_cosmics(x, 3, 5, 3, effective_gain=2.0, readnoise=5.0)
+            return x, out
+
+        def fwd_bwd():
+            x, out = fwd()
+            (w * out).sum().backward()
+            return x.grad
+
+        for _ in range(2):
+            fwd_bwd()
+        torch.cuda.synchronize()
+        reps = max(2, args.steps // 2)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fwd()
+        torch.cuda.synchronize()
+        f_ms = 1e3 * (time.perf_counter() - t0) / reps
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fwd_bwd()
+        torch.cuda.synchronize()
+        fb_ms = 1e3 * (time.perf_counter() - t0) / reps
+        result["fwd_bwd"] = {"value": round(mp4 / (fb_ms / 1e3), 2), "unit": "Mpix/s",
+                             "forward_mpix_s": round(mp4 / (f_ms / 1e3), 2),
+                             "ms_forward": round(f_ms, 3), "ms_forward_backward": round(fb_ms, 3),
+                             "ratio": round(fb_ms / f_ms, 3), "frames": "8 x 1024x1024"}
+        log(f"fwd {f_ms:.3f} ms, fwd+bwd {fb_ms:.3f} ms")
+
+    if not args.no_cpu:
+        import oracle
+
+        cores, model = cpu_info()
+        sample = [b for b in batch if b["data"].numel() == 1 << 20][:1] or batch[:1]
+        frames = [dict(data=b["data"].cpu().numpy(), error=b["error"].cpu().numpy()) for b in sample]
+        spx = sum(f["data"].size for f in frames) / 1e6
+        log(f"cpu baseline: oracle, 1 thread, {spx:.2f} Mpx")
+        t0 = time.perf_counter()
+        oracle.clean_frames(frames, threads=1)
+        c_s = time.perf_counter() - t0
+        shape = "x".join(map(str, frames[0]["data"].shape))
+        result["cpu_baseline"] = {"value": round(spx / c_s, 4), "unit": "Mpix/s", "cores": 1,
+                                  "kind": "port", "cpu": model, "host_cores": cores,
+                                  "sample": f"{len(frames)} frame(s) of {shape} from the same ragged "
+                                            f"batch (seed 3), error map, maxiter 4; {c_s:.1f} s"}
+        log(f"cpu: {spx / c_s:.3f} Mpix/s")
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

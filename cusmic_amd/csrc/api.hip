@@ -1,0 +1,528 @@
+/*
+ * cusmic C ABI on HIP (include/cusmic.h, include/cusmic_amd.h).
+ *
+ * Host side of the cleaner: argument checks with the reference's messages and
+ * status codes (src/api.cu:47-114), the host/device/batch entry points
+ * (src/api.cu:155-192) and the ragged-batch/adjoint extensions. One call
+ * enqueues the whole loop on the caller's stream and synchronises once at the
+ * end to read the device counters (errors, passes).
+ */
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <vector>
+
+#include "../../include/cusmic.h"
+#include "../../include/cusmic_amd.h"
+#include "kernels.hip"
+
+#ifndef CUSMIC_VERSION
+#define CUSMIC_VERSION "cusmic-amd 0.1.0"
+#endif
+
+using namespace cusmic;
+
+namespace {
+
+void
+hip_check(hipError_t status)
+{
+	if (status == hipErrorOutOfMemory || status == hipErrorMemoryAllocation)
+		throw std::bad_alloc();
+	if (status != hipSuccess)
+		throw std::runtime_error(hipGetErrorString(status));
+}
+
+/* Exception firewall: no C++ exception crosses the ABI (src/api.cu:83-105). */
+template <class Func>
+int
+guarded(Func run, char *msg, size_t len)
+{
+	if (msg && len)
+		msg[0] = 0;
+	try {
+		run();
+		return CUSMIC_OK;
+	} catch (const std::invalid_argument &e) {
+		if (msg && len)
+			std::snprintf(msg, len, "%s", e.what());
+		return CUSMIC_INVALID;
+	} catch (const std::bad_alloc &e) {
+		if (msg && len)
+			std::snprintf(msg, len, "%s", e.what());
+		return CUSMIC_NO_MEMORY;
+	} catch (const std::exception &e) {
+		if (msg && len)
+			std::snprintf(msg, len, "%s", e.what());
+		return CUSMIC_CUDA_ERROR;
+	}
+}
+
+/* Settings and shapes (src/api.cu:47-81); same_shape selects the batch rule. */
+size_t
+check_arguments(const cusmic_image *ims, size_t nf, const cusmic_options *o, const double *clean,
+	const uint8_t *mask, bool same_shape, const double *fg = nullptr,
+	const double *frn = nullptr)
+{
+	if (!ims || !o || !clean || !mask || !nf || nf > 65535)
+		throw std::invalid_argument("invalid images, batch size or output buffers");
+	if (o->maxiter < 0 || o->border < 0 || o->border > CUSMIC_WRAP || !std::isfinite(o->contrast) ||
+		o->contrast < 0 || !std::isfinite(o->cr_threshold) || o->cr_threshold < 0 ||
+		!std::isfinite(o->neighbor_threshold) || o->neighbor_threshold < 0)
+		throw std::invalid_argument("invalid detection settings");
+	size_t total = 0;
+	for (size_t f = 0; f < nf; ++f) {
+		const cusmic_image &im = ims[f];
+		size_t w = im.width, h = im.height;
+		if (!w || !h || w > size_t(INT_MAX) / 4 / h)
+			throw std::invalid_argument("invalid image dimensions");
+		if (!im.data || (same_shape && (w != ims[0].width || h != ims[0].height)))
+			throw std::invalid_argument("frames must have data and the same shape");
+		double g = fg ? fg[f] : o->gain, rn = frn ? frn[f] : o->readnoise;
+		if (!im.error) {
+			bool needs_gain = o->maxiter || !std::isnan(g);
+			if (!im.gain && needs_gain && (!std::isfinite(g) || g <= 0))
+				throw std::invalid_argument("positive gain or error is required");
+			if (!im.readnoise && (!std::isfinite(rn) || rn < 0))
+				throw std::invalid_argument("read noise must be finite and nonnegative");
+		}
+		if (!im.background && !std::isnan(o->background) && !std::isfinite(o->background))
+			throw std::invalid_argument("background must be finite");
+		total += w * h;
+	}
+	if (total > size_t(UINT_MAX) || total > SIZE_MAX / sizeof(double))
+		throw std::invalid_argument("batch is too large");
+	return total;
+}
+
+/* Host map contents (src/api.cu:107-114). */
+void
+check_values(const double *data, size_t n, const char *name, int minimum)
+{
+	for (size_t i = 0; data && i < n; ++i)
+		if (!std::isfinite(data[i]) || (minimum == 1 && data[i] <= 0) ||
+			(minimum == 0 && data[i] < 0))
+			throw std::invalid_argument(name);
+}
+
+/* Stream-ordered device scratch released at scope exit. */
+struct scratch {
+	void *ptr = nullptr;
+	hipStream_t stream;
+	scratch(size_t bytes, hipStream_t s) : stream(s)
+	{
+		if (bytes)
+			hip_check(hipMallocAsync(&ptr, bytes, s));
+	}
+	~scratch()
+	{
+		if (ptr)
+			(void)hipFreeAsync(ptr, stream);
+	}
+	scratch(const scratch &) = delete;
+	scratch &operator=(const scratch &) = delete;
+};
+
+size_t
+align_up(size_t v, size_t a)
+{
+	return (v + a - 1) / a * a;
+}
+
+int
+device_cus()
+{
+	int dev = 0, n = 0;
+	hip_check(hipGetDevice(&dev));
+	hip_check(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+	return n > 0 ? n : 1;
+}
+
+/* Frame table of a batch: offsets and tile numbering. */
+std::vector<Frame>
+frame_table(const cusmic_image *ims, size_t nf, const cusmic_options &o, const double *fg,
+	const double *frn, int *ntiles_out)
+{
+	std::vector<Frame> frames(nf);
+	long long off = 0;
+	int tiles = 0;
+	for (size_t f = 0; f < nf; ++f) {
+		const cusmic_image &im = ims[f];
+		Frame &fr = frames[f];
+		fr.data = im.data;
+		fr.error = im.error;
+		fr.gain = im.error ? nullptr : im.gain;
+		fr.readnoise = im.error ? nullptr : im.readnoise;
+		fr.background = im.background;
+		fr.mask = im.mask;
+		fr.g = fg ? fg[f] : o.gain;
+		fr.rn = frn ? frn[f] : o.readnoise;
+		fr.w = int(im.width);
+		fr.h = int(im.height);
+		fr.off = off;
+		fr.tiles_x = (fr.w + TILE_W - 1) / TILE_W;
+		fr.tiles_y = (fr.h + TILE_H - 1) / TILE_H;
+		fr.tile0 = tiles;
+		if ((long long)tiles + (long long)fr.tiles_x * fr.tiles_y > INT_MAX)
+			throw std::invalid_argument("batch is too large");
+		tiles += fr.tiles_x * fr.tiles_y;
+		off += (long long)fr.w * fr.h;
+	}
+	*ntiles_out = tiles;
+	return frames;
+}
+
+/* The whole call: prologue, hole fill, maxiter detection/replacement passes
+ * and epilogue (src/cleaner.cuh:144-187), all device-driven. */
+/* Optional per-kernel timing with events on the call's stream. */
+struct timer {
+	hipStream_t stream;
+	bool on;
+	std::vector<hipEvent_t> events;
+	std::vector<int> kinds;
+	explicit timer(hipStream_t s, bool enabled) : stream(s), on(enabled) {}
+	~timer()
+	{
+		for (auto e : events)
+			(void)hipEventDestroy(e);
+	}
+	void
+	mark(int kind)
+	{
+		if (!on)
+			return;
+		hipEvent_t e;
+		hip_check(hipEventCreate(&e));
+		hip_check(hipEventRecord(e, stream));
+		events.push_back(e);
+		kinds.push_back(kind);
+	}
+	/* kind k of event i times the interval from event i-1 to i */
+	void
+	collect(double *out)
+	{
+		for (size_t i = 1; i < events.size(); ++i) {
+			float ms = 0;
+			hip_check(hipEventElapsedTime(&ms, events[i - 1], events[i]));
+			if (kinds[i] >= 0)
+				out[kinds[i]] += ms;
+		}
+	}
+};
+
+void
+clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const double *fg,
+	const double *frn, int laplacian, double *clean, uint8_t *mask, int32_t *donors,
+	int32_t *passes, hipStream_t stream, double *profile = nullptr)
+{
+	int ntiles = 0;
+	std::vector<Frame> frames = frame_table(ims, nf, o, fg, frn, &ntiles);
+	size_t total = 0;
+	for (const auto &fr : frames)
+		total += size_t(fr.w) * fr.h;
+	int maxiter = o.maxiter;
+
+	/* one scratch block: frame table | counters (zeroed) | flags | list */
+	size_t n_counts = nf * (sizeof(Counts) / 4);
+	size_t n_todo = size_t(std::max(maxiter, 1)) * nf;
+	size_t n_queue = size_t(std::max(maxiter, 1));
+	size_t n_nlist = 2 + size_t(std::max(maxiter, 1));
+	size_t n_dirty = 2 * size_t(ntiles);
+	size_t words = n_counts + n_todo + n_queue + n_nlist + 1 + n_dirty;
+	size_t off_frames = 0;
+	size_t off_words = align_up(nf * sizeof(Frame), 256);
+	size_t off_dlist = align_up(off_words + words * 4, 256);
+	size_t off_work = align_up(off_dlist + n_dirty * 4, 256);
+	size_t off_flags = align_up(off_work + n_queue * 8, 256);
+	size_t off_list = align_up(off_flags + (maxiter ? total : 0), 256);
+	size_t bytes = off_list + (maxiter ? total * sizeof(unsigned long long) : 0);
+	scratch mem(bytes, stream);
+	char *base = static_cast<char *>(mem.ptr);
+	hip_check(hipMemcpyAsync(base + off_frames, frames.data(), nf * sizeof(Frame),
+		hipMemcpyHostToDevice, stream));
+
+	Params p{};
+	p.frames = reinterpret_cast<const Frame *>(base + off_frames);
+	p.nf = int(nf);
+	p.ntiles = ntiles;
+	p.maxiter = maxiter;
+	p.mode = o.border;
+	p.laplacian = laplacian;
+	p.contrast = o.contrast;
+	p.threshold = o.cr_threshold;
+	p.neighbor = o.neighbor_threshold;
+	p.bg = o.background;
+	p.clean = clean;
+	p.mask = mask;
+	p.donors = donors;
+	unsigned int *w32 = reinterpret_cast<unsigned int *>(base + off_words);
+	p.counts = reinterpret_cast<Counts *>(w32);
+	p.todo = reinterpret_cast<int *>(w32 + n_counts);
+	p.queue = w32 + n_counts + n_todo;
+	p.nlist = p.queue + n_queue;
+	p.status = reinterpret_cast<int *>(p.nlist + n_nlist);
+	p.dirty = reinterpret_cast<unsigned int *>(p.status + 1);
+	p.dirty_list = reinterpret_cast<unsigned int *>(base + off_dlist);
+	p.work = reinterpret_cast<unsigned long long *>(base + off_work);
+	p.flags = reinterpret_cast<uint8_t *>(base + off_flags);
+	p.list = reinterpret_cast<unsigned long long *>(base + off_list);
+
+	if (!maxiter) {
+		hipLaunchKernelGGL(copy_without_detection, dim3(ntiles), dim3(BLOCK), 0, stream, p);
+		hip_check(hipGetLastError());
+		hip_check(hipStreamSynchronize(stream));
+		return;
+	}
+
+	int cus = device_cus();
+	timer tm(stream, profile != nullptr);
+	hipLaunchKernelGGL(zero_words, dim3(unsigned(std::min<size_t>((words + 255) / 256, 4096))),
+		dim3(256), 0, stream, w32, words);
+	hipLaunchKernelGGL(zero_words, dim3(1), dim3(256), 0, stream,
+		reinterpret_cast<unsigned int *>(p.work), n_queue * 2);
+	tm.mark(-1);
+	hipLaunchKernelGGL(prologue, dim3(ntiles), dim3(BLOCK), 0, stream, p);
+	tm.mark(CUSMIC_PROFILE_PROLOGUE_MS);
+	unsigned replace_grid = unsigned(cus) * 8;
+	hipLaunchKernelGGL(replace_pass, dim3(replace_grid), dim3(BLOCK), 0, stream, p, 0, 1);
+	tm.mark(CUSMIC_PROFILE_REPLACE_MS);
+	unsigned detect_grid = unsigned(std::min(ntiles, cus * 2));
+	for (int it = 0; it < maxiter; ++it) {
+		unsigned grid = it == 0 ? detect_grid : unsigned(cus * 2);
+		hipLaunchKernelGGL(detect, dim3(grid), dim3(BLOCK), 0, stream, p, it);
+		tm.mark(CUSMIC_PROFILE_DETECT_MS);
+		hipLaunchKernelGGL(replace_pass, dim3(replace_grid), dim3(BLOCK), 0, stream, p, it, 0);
+		tm.mark(CUSMIC_PROFILE_REPLACE_MS);
+	}
+	hipLaunchKernelGGL(epilogue, dim3(ntiles), dim3(BLOCK), 0, stream, p);
+	tm.mark(CUSMIC_PROFILE_EPILOGUE_MS);
+	hip_check(hipGetLastError());
+
+	std::vector<int> host(n_todo + 1);
+	std::vector<unsigned long long> work(n_queue);
+	unsigned int nflagged = 0;
+	hip_check(hipMemcpyAsync(host.data(), p.todo, n_todo * sizeof(int), hipMemcpyDeviceToHost,
+		stream));
+	hip_check(hipMemcpyAsync(host.data() + n_todo, p.status, sizeof(int), hipMemcpyDeviceToHost,
+		stream));
+	if (profile) {
+		hip_check(hipMemcpyAsync(work.data(), p.work, n_queue * 8, hipMemcpyDeviceToHost, stream));
+		hip_check(hipMemcpyAsync(&nflagged, p.nlist, 4, hipMemcpyDeviceToHost, stream));
+	}
+	hip_check(hipStreamSynchronize(stream));
+	if (profile) {
+		for (int k = 0; k < CUSMIC_PROFILE_SIZE; ++k)
+			profile[k] = 0;
+		tm.collect(profile);
+		profile[CUSMIC_PROFILE_DETECT_LAUNCHES] = maxiter;
+		for (auto v : work)
+			profile[CUSMIC_PROFILE_DETECT_PIXELS] += double(v);
+		profile[CUSMIC_PROFILE_FLAGGED] = nflagged;
+	}
+	if (host[n_todo] & ERR_NO_DONORS)
+		throw std::invalid_argument("no finite replacement donors");
+	if (passes)
+		for (size_t f = 0; f < nf; ++f) {
+			int k = 0;
+			for (int it = 0; it < maxiter; ++it) {
+				++k;
+				if (!host[size_t(it) * nf + f])
+					break;
+			}
+			passes[f] = k;
+		}
+}
+
+struct host_result {
+	std::unique_ptr<double[]> clean;
+	std::unique_ptr<uint8_t[]> mask;
+	explicit host_result(size_t n) : clean(new double[n]), mask(new uint8_t[n]) {}
+};
+
+/* Host path (src/api.cu:116-153): maxiter = 0 stays on the CPU. */
+host_result
+clean_host(const cusmic_image &im, const cusmic_options &o, size_t n)
+{
+	host_result out(n);
+	if (!o.maxiter) {
+		std::memcpy(out.clean.get(), im.data, n * sizeof(double));
+		std::memset(out.mask.get(), 0, n);
+		if (im.background || !std::isnan(o.background))
+			for (size_t i = 0; i < n; ++i) {
+				if (!std::isfinite(im.data[i]))
+					continue;
+				double bg = im.background ? im.background[i] : o.background;
+				out.clean[i] += bg;
+				out.clean[i] -= bg;
+			}
+		return out;
+	}
+	hipStream_t stream = nullptr;
+	const double *gmap = im.error ? nullptr : im.gain;
+	const double *rnmap = im.error ? nullptr : im.readnoise;
+	size_t planes = 2 + (im.error != nullptr) + (gmap != nullptr) + (rnmap != nullptr) +
+			(im.background != nullptr);
+	scratch mem(planes * n * sizeof(double) + 2 * n + 512, stream);
+	char *base = static_cast<char *>(mem.ptr);
+	double *d = reinterpret_cast<double *>(base);
+	double *next = d;
+	auto plane = [&](const double *src) -> const double * {
+		if (!src)
+			return nullptr;
+		double *dst = next;
+		next += n;
+		hip_check(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, stream));
+		return dst;
+	};
+	cusmic_image gpu{};
+	gpu.data = plane(im.data);
+	gpu.error = plane(im.error);
+	gpu.gain = plane(gmap);
+	gpu.readnoise = plane(rnmap);
+	gpu.background = plane(im.background);
+	double *clean = next;
+	next += n;
+	uint8_t *mask = reinterpret_cast<uint8_t *>(next);
+	uint8_t *excl = mask + align_up(n, 256);
+	if (im.mask) {
+		hip_check(hipMemcpyAsync(excl, im.mask, n, hipMemcpyHostToDevice, stream));
+		gpu.mask = excl;
+	}
+	gpu.width = im.width;
+	gpu.height = im.height;
+	clean_images(&gpu, 1, o, nullptr, nullptr, CUSMIC_LAPLACIAN_EXACT, clean, mask, nullptr,
+		nullptr, stream);
+	hip_check(hipMemcpyAsync(out.clean.get(), clean, n * sizeof(double), hipMemcpyDeviceToHost,
+		stream));
+	hip_check(hipMemcpyAsync(out.mask.get(), mask, n, hipMemcpyDeviceToHost, stream));
+	hip_check(hipStreamSynchronize(stream));
+	return out;
+}
+
+} // namespace
+
+extern "C" const char *
+cusmic_version(void)
+{
+	return CUSMIC_VERSION;
+}
+
+extern "C" int
+cusmic_cuda_version(void)
+{
+	return HIP_VERSION;
+}
+
+extern "C" void
+cusmic_default_options(cusmic_options *options)
+{
+	if (options)
+		*options = {3, 5, 3, NAN, 0, NAN, 4, CUSMIC_MIRROR};
+}
+
+extern "C" int
+remove_cosmics(const cusmic_image *im, const cusmic_options *o, double *clean, uint8_t *mask,
+	char *msg, size_t len)
+{
+	return guarded([&] {
+		size_t n = check_arguments(im, 1, o, clean, mask, true);
+		if (im->error)
+			check_values(im->error, n, "error must be finite and positive", 1);
+		else {
+			check_values(im->gain, n, "gain must be finite and positive", 1);
+			check_values(im->readnoise, n, "read noise must be finite and nonnegative", 0);
+		}
+		check_values(im->background, n, "background must be finite", -1);
+		auto out = clean_host(*im, *o, n);
+		std::memcpy(clean, out.clean.get(), n * sizeof(double));
+		std::memcpy(mask, out.mask.get(), n);
+	}, msg, len);
+}
+
+extern "C" int
+remove_cosmics_batch_device(const cusmic_image *ims, size_t nf, const cusmic_options *o,
+	double *clean, uint8_t *mask, void *stream, char *msg, size_t len)
+{
+	return guarded([&] {
+		check_arguments(ims, nf, o, clean, mask, true);
+		clean_images(ims, nf, *o, nullptr, nullptr, CUSMIC_LAPLACIAN_EXACT, clean, mask, nullptr,
+			nullptr, static_cast<hipStream_t>(stream));
+	}, msg, len);
+}
+
+extern "C" int
+remove_cosmics_device(const cusmic_image *im, const cusmic_options *o, double *clean,
+	uint8_t *mask, void *stream, char *msg, size_t len)
+{
+	return remove_cosmics_batch_device(im, 1, o, clean, mask, stream, msg, len);
+}
+
+extern "C" int
+remove_cosmics_ragged_device(const cusmic_image *ims, size_t nf, const cusmic_options *o,
+	const cusmic_ragged_options *ext, double *clean, uint8_t *mask, void *stream, char *msg,
+	size_t len)
+{
+	return guarded([&] {
+		const double *fg = ext ? ext->frame_gain : nullptr;
+		const double *frn = ext ? ext->frame_readnoise : nullptr;
+		int lap = ext ? ext->laplacian : CUSMIC_LAPLACIAN_EXACT;
+		if (lap != CUSMIC_LAPLACIAN_EXACT && lap != CUSMIC_LAPLACIAN_CLOSE)
+			throw std::invalid_argument("invalid Laplacian order");
+		check_arguments(ims, nf, o, clean, mask, false, fg, frn);
+		clean_images(ims, nf, *o, fg, frn, lap, clean, mask, ext ? ext->donors : nullptr,
+			ext ? ext->passes : nullptr, static_cast<hipStream_t>(stream),
+			ext ? ext->profile : nullptr);
+	}, msg, len);
+}
+
+extern "C" int
+cusmic_adjoint_device(const cusmic_image *ims, size_t nf, const uint8_t *mask,
+	const int32_t *donors, const double *grad_clean, double *grad_data, void *stream, char *msg,
+	size_t len)
+{
+	return guarded([&] {
+		if (!ims || !nf || nf > 65535 || !mask || !donors || !grad_clean || !grad_data)
+			throw std::invalid_argument("invalid images, batch size or output buffers");
+		for (size_t f = 0; f < nf; ++f)
+			if (!ims[f].width || !ims[f].height ||
+				ims[f].width > size_t(INT_MAX) / 4 / ims[f].height)
+				throw std::invalid_argument("invalid image dimensions");
+		hipStream_t s = static_cast<hipStream_t>(stream);
+		cusmic_options o;
+		cusmic_default_options(&o);
+		int ntiles = 0;
+		std::vector<Frame> frames = frame_table(ims, nf, o, nullptr, nullptr, &ntiles);
+		size_t off_reach = align_up(nf * sizeof(Frame), 256);
+		scratch mem(off_reach + nf * sizeof(int), s);
+		char *base = static_cast<char *>(mem.ptr);
+		hip_check(hipMemcpyAsync(base, frames.data(), nf * sizeof(Frame), hipMemcpyHostToDevice, s));
+		int *reach = reinterpret_cast<int *>(base + off_reach);
+		hipLaunchKernelGGL(zero_words, dim3(1), dim3(256), 0, s,
+			reinterpret_cast<unsigned int *>(reach), nf);
+		const Frame *dframes = reinterpret_cast<const Frame *>(base);
+		hipLaunchKernelGGL(donor_reach, dim3(ntiles), dim3(BLOCK), 0, s, dframes, int(nf), mask,
+			donors, reach);
+		hipLaunchKernelGGL(adjoint_gather, dim3(ntiles), dim3(BLOCK), 0, s, dframes, int(nf), mask,
+			donors, reach, grad_clean, grad_data);
+		hip_check(hipGetLastError());
+		hip_check(hipStreamSynchronize(s));
+	}, msg, len);
+}
+
+extern "C" int
+cusmic_selftest_arith(const double *a, const double *b, double *quotient, double *root, size_t n,
+	void *stream)
+{
+	return guarded([&] {
+		hipStream_t s = static_cast<hipStream_t>(stream);
+		hipLaunchKernelGGL(selftest_arith, dim3(256), dim3(256), 0, s, a, b, quotient, root, n);
+		hip_check(hipGetLastError());
+		hip_check(hipStreamSynchronize(s));
+	}, nullptr, 0);
+}

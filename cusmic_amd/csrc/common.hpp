@@ -1,0 +1,139 @@
+/*
+ * Device-side data layout shared by the cleaning kernels (gfx950).
+ *
+ * A call cleans a batch of frames of individual shapes. Every per-pixel plane
+ * (clean, flags, mask, donors) is the frames' pixels concatenated in
+ * descriptor order; frame f starts at Frame::off. Frames are cut into
+ * TILE_W x TILE_H tiles numbered frame-major, row-major; Frame::tile0 is the
+ * first tile of frame f.
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../include/cusmic.h"
+
+namespace cusmic {
+
+constexpr int TILE_W = 64;
+constexpr int TILE_H = 32;
+constexpr int BLOCK = 256;
+
+/* Per-pixel state bits of the flags plane. */
+constexpr uint8_t HOLE = 1;    /* input pixel is nonfinite */
+constexpr uint8_t EXCL = 2;    /* hole or masked: never a seed or donor */
+constexpr uint8_t CRAY = 4;    /* flagged as cosmic ray (the output mask) */
+
+constexpr double FINE_FLOOR = 0.01;   /* src/filters.cuh:21 */
+constexpr double SIGNAL_FLOOR = 1e-5; /* src/filters.cuh:22 */
+
+/* Detection reach: a change of the clean image at one pixel can change the
+ * detection result only within this Chebyshev radius (Laplacian 1 + median 2
+ * of the significance + two growth steps 2, or median 1 + median 3 of the fine
+ * structure + 2). */
+constexpr int REACH = 6;
+
+struct Frame {
+	const double *data, *error, *gain, *readnoise, *background;
+	const uint8_t *mask;
+	double g, rn;          /* scalar gain and read noise of this frame */
+	long long off;         /* first pixel in the batch planes */
+	int w, h;
+	int tile0, tiles_x, tiles_y;
+};
+
+/* Device counters of one call; zeroed by reset_counters. */
+struct Counts {
+	int holes, allowed, donors, error;
+};
+
+struct Params {
+	const Frame *frames;
+	int nf, ntiles, maxiter, mode, laplacian;
+	double contrast, threshold, neighbor, bg;  /* bg: scalar background or NaN */
+	double *clean;
+	uint8_t *flags;
+	uint8_t *mask;
+	int32_t *donors;                 /* 2 per pixel or null */
+	unsigned long long *list;        /* hole list, then the flagged-pixel list */
+	Counts *counts;                  /* per frame */
+	int *todo;                       /* [maxiter][nf] new pixels per pass */
+	unsigned int *queue;             /* [maxiter] tile queue heads */
+	unsigned int *nlist;             /* [0] list length, [1] hole count, [2..] dirty counts */
+	unsigned int *dirty;             /* [2][ntiles] dirty flags */
+	unsigned int *dirty_list;        /* [2][ntiles] dirty tile lists */
+	int *status;                     /* [0] error bits */
+	unsigned long long *work;        /* [maxiter] in-frame pixels of processed tiles */
+};
+
+constexpr int ERR_NO_DONORS = 1;
+
+/* scipy.ndimage border mapping of coordinate x on an axis of n samples;
+ * -1 means a CONSTANT-mode zero (src/filters.cuh:24-44). */
+__device__ __forceinline__ int
+border_index(int x, int n, int mode)
+{
+	if (x >= 0 && x < n)
+		return x;
+	if (mode == CUSMIC_CONSTANT)
+		return -1;
+	if (mode == CUSMIC_NEAREST)
+		return x < 0 ? 0 : n - 1;
+	if (n == 1)
+		return 0;
+	if (mode == CUSMIC_WRAP) {
+		x %= n;
+		return x < 0 ? x + n : x;
+	}
+	int p = mode == CUSMIC_REFLECT ? 2 * n : 2 * (n - 1);
+	x %= p;
+	if (x < 0)
+		x += p;
+	if (x < n)
+		return x;
+	return mode == CUSMIC_REFLECT ? p - 1 - x : p - x;
+}
+
+/* Order-preserving key: -0 below +0 (src/replace.cuh:228-241). */
+__device__ __forceinline__ unsigned long long
+order_key(double v)
+{
+	unsigned long long b = (unsigned long long)__double_as_longlong(v);
+	return (b >> 63) ? ~b : b ^ (1ull << 63);
+}
+
+__device__ __forceinline__ double
+key_value(unsigned long long k)
+{
+	return __longlong_as_double((long long)((k >> 63) ? k ^ (1ull << 63) : ~k));
+}
+
+/* src/replace.cuh:243-249 */
+__device__ __forceinline__ double
+midpoint(double lo, double hi)
+{
+	if (fabs(lo) < 1 && fabs(hi) < 1)
+		return (lo + hi) / 2;
+	return lo / 2 + hi / 2;
+}
+
+__device__ __forceinline__ int
+find_frame(const Frame *frames, int nf, int tile)
+{
+	int lo = 0, hi = nf - 1;
+	while (lo < hi) {
+		int mid = (lo + hi + 1) >> 1;
+		if (frames[mid].tile0 <= tile)
+			lo = mid;
+		else
+			hi = mid - 1;
+	}
+	return lo;
+}
+
+__device__ __forceinline__ unsigned long long
+list_entry(int frame, int index)
+{
+	return ((unsigned long long)(unsigned)frame << 32) | (unsigned)index;
+}
+
+} // namespace cusmic

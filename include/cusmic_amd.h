@@ -1,0 +1,76 @@
+/*
+ * Extensions of the cusmic C ABI that have no counterpart in the reference
+ * (src/cusmic.h). The six reference entry points are declared in cusmic.h and
+ * behave as there; everything here is additive.
+ */
+#ifndef CUSMIC_AMD_H
+#define CUSMIC_AMD_H
+
+#include "cusmic.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Laplacian accumulation order. EXACT follows L.A.Cosmic 1.4.0 (scipy's
+ * convolution order on the 2x-replicated frame); CLOSE is the expression of the
+ * reference CUDA kernel (src/filters.cuh:103-104). Both gave the saved
+ * reference's mask; they can differ by rounding near a threshold. */
+enum cusmic_laplacian { CUSMIC_LAPLACIAN_EXACT = 0, CUSMIC_LAPLACIAN_CLOSE = 1 };
+
+/* Per-call extras of remove_cosmics_ragged_device; zero-initialise and set what
+ * is needed. All pointers may be NULL. */
+struct cusmic_ragged_options {
+	const double *frame_gain;      /* host, nf values: per-frame scalar gain */
+	const double *frame_readnoise; /* host, nf values: per-frame scalar read noise */
+	int32_t *donors;    /* device, 2 per pixel: final median donors (in-frame
+	                       indices) of flagged pixels, -1 elsewhere */
+	int32_t *passes;    /* host, nf values: detection passes each frame ran */
+	int laplacian;      /* enum cusmic_laplacian */
+	double *profile;    /* host, CUSMIC_PROFILE_SIZE values, or NULL: when set,
+	                       the call times its kernels with hipEvents on the
+	                       caller's stream (see enum cusmic_profile) */
+};
+
+/* Slots of cusmic_ragged_options.profile. Times are summed over launches in
+ * milliseconds; DETECT_PIXELS counts the in-frame pixels of every tile the
+ * detection passes processed. */
+enum cusmic_profile {
+	CUSMIC_PROFILE_DETECT_MS, CUSMIC_PROFILE_REPLACE_MS, CUSMIC_PROFILE_PROLOGUE_MS,
+	CUSMIC_PROFILE_EPILOGUE_MS, CUSMIC_PROFILE_DETECT_LAUNCHES, CUSMIC_PROFILE_DETECT_PIXELS,
+	CUSMIC_PROFILE_FLAGGED, CUSMIC_PROFILE_SIZE
+};
+
+/* Ragged batch: 1..65535 contiguous device frames of individual shapes,
+ * cleaned in one call on the caller's stream (persistent tile-queue kernels,
+ * no host round trip inside the loop). Outputs are the frames concatenated in
+ * descriptor order. Per-frame scalars in ext replace options->gain/readnoise.
+ * Same validation, status codes and completion semantics as
+ * remove_cosmics_batch_device (src/cusmic.h:80-85), which is the special
+ * case of equal shapes. No counterpart in the reference. */
+int remove_cosmics_ragged_device(const struct cusmic_image *ims, size_t nf,
+	const struct cusmic_options *options, const struct cusmic_ragged_options *ext,
+	double *clean, uint8_t *mask, void *stream, char *msg, size_t message_size);
+
+/* Adjoint of the cleaning map with the detection mask held fixed:
+ * grad_data = grad_clean on unflagged pixels, plus, for every flagged pixel,
+ * grad_clean scattered to its recorded donors (weight 1 for an odd donor count,
+ * 1/2 each for the two middle donors of an even count). Written as a
+ * deterministic gather: no atomics, fixed summation order. The gradient with
+ * respect to the background is exactly zero and is not computed. Frames,
+ * mask and donors are laid out as returned by remove_cosmics_ragged_device;
+ * only width/height of ims are read. No counterpart in the reference. */
+int cusmic_adjoint_device(const struct cusmic_image *ims, size_t nf, const uint8_t *mask,
+	const int32_t *donors, const double *grad_clean, double *grad_data, void *stream, char *msg,
+	size_t message_size);
+
+/* Diagnostic: quotient[i] = a[i] / b[i] and root[i] = sqrt(a[i]) computed on
+ * the GPU with the kernels' own compile flags, for bitwise checks of IEEE
+ * rounding against the host. Device arrays. No counterpart in the reference. */
+int cusmic_selftest_arith(const double *a, const double *b, double *quotient, double *root,
+	size_t n, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
