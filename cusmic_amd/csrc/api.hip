@@ -239,7 +239,8 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	size_t off_words = align_up(nf * sizeof(Frame), 256);
 	size_t off_dlist = align_up(off_words + words * 4, 256);
 	size_t off_work = align_up(off_dlist + n_dirty * 4, 256);
-	size_t off_flags = align_up(off_work + n_queue * 8, 256);
+	size_t off_tframe = align_up(off_work + n_queue * 8, 256);
+	size_t off_flags = align_up(off_tframe + size_t(ntiles) * 4, 256);
 	size_t off_list = align_up(off_flags + (maxiter ? total : 0), 256);
 	size_t bytes = off_list + (maxiter ? total * sizeof(unsigned long long) : 0);
 	scratch mem(bytes, stream);
@@ -270,6 +271,7 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	p.dirty = reinterpret_cast<unsigned int *>(p.status + 1);
 	p.dirty_list = reinterpret_cast<unsigned int *>(base + off_dlist);
 	p.work = reinterpret_cast<unsigned long long *>(base + off_work);
+	p.tile_frame = reinterpret_cast<int *>(base + off_tframe);
 	p.flags = reinterpret_cast<uint8_t *>(base + off_flags);
 	p.list = reinterpret_cast<unsigned long long *>(base + off_list);
 

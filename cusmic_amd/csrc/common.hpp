@@ -63,6 +63,7 @@ struct Params {
 	unsigned int *dirty_list;        /* [2][ntiles] dirty tile lists */
 	int *status;                     /* [0] error bits */
 	unsigned long long *work;        /* [maxiter] in-frame pixels of processed tiles */
+	int *tile_frame;                 /* [ntiles] frame of each tile (set by the prologue) */
 };
 
 constexpr int ERR_NO_DONORS = 1;

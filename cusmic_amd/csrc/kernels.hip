@@ -134,6 +134,8 @@ prologue(Params p)
 	TilePixels t = tile_of(fr, tile);
 	bool bg_map = fr.background != nullptr, bg = bg_map || !isnan(p.bg);
 	int holes = 0, allowed = 0, donors = 0;
+	if (!threadIdx.x)
+		p.tile_frame[tile] = f;
 	int x = t.x0 + (threadIdx.x & 63);
 	for (int r = threadIdx.x >> 6; r < TILE_H; r += 4) {
 		int y = t.y0 + r;
@@ -367,22 +369,27 @@ replace_pixel(const Params &p, const Frame &fr, int i, int32_t *donors)
 	const double *clean = p.clean + fr.off;
 	const uint8_t *flags = p.flags + fr.off;
 	int w = fr.w, h = fr.h, x = i % w, y = i / w;
+	/* flags and values of the 24 neighbours are loaded together (one memory
+	 * round trip); a value read at a flagged neighbour, which another lane may
+	 * be rewriting, is discarded */
 	double v[24];
-	int n = 0, neg0 = 0, k = 0;
+	uint8_t fl[24];
+	int n = 0, neg0 = 0;
 #pragma unroll
-	for (int dy = -2; dy <= 2; ++dy)
+	for (int k = 0; k < 24; ++k) {
+		int o = k < 12 ? k : k + 1, qx = x + o % 5 - 2, qy = y + o / 5 - 2;
+		bool in = qx >= 0 && qx < w && qy >= 0 && qy < h;
+		int q = in ? qy * w + qx : i;
+		fl[k] = in ? flags[q] : EXCL;
+		v[k] = clean[q];
+	}
 #pragma unroll
-		for (int dx = -2; dx <= 2; ++dx) {
-			if (!dx && !dy)
-				continue;
-			int qx = x + dx, qy = y + dy;
-			double val = INFINITY;
-			bool ok = qx >= 0 && qx < w && qy >= 0 && qy < h &&
-				  is_donor(flags, clean, qy * w + qx, val);
-			v[k++] = ok ? val : INFINITY;
-			n += ok;
-			neg0 += ok && (val < 0 || (val == 0.0 && signbit(val)));
-		}
+	for (int k = 0; k < 24; ++k) {
+		bool ok = !(fl[k] & (EXCL | CRAY)) && isfinite(v[k]);
+		v[k] = ok ? v[k] : INFINITY;
+		n += ok;
+		neg0 += ok && (v[k] < 0 || (v[k] == 0.0 && signbit(v[k])));
+	}
 	double lo, hi;
 	int r = 2;
 	int x0 = max(0, x - 2), x1 = min(w - 1, x + 2), y0 = max(0, y - 2), y1 = min(h - 1, y + 2);
@@ -491,6 +498,9 @@ constexpr int SW = TILE_W + 2 * HS, SH = TILE_H + 2 * HS;  /* 72 x 40 */
 constexpr int PW = TILE_W + 2 * HP, PH = TILE_H + 2 * HP;  /* 68 x 36 */
 constexpr int GW = TILE_W + 2, GH = TILE_H + 2;            /* 66 x 34 */
 constexpr int PER_THREAD = (PW * PH + BLOCK - 1) / BLOCK;  /* 10 */
+constexpr int C_PER = (CW * CH + BLOCK - 1) / BLOCK;       /* 14 */
+constexpr int S_PER = (SW * SH + BLOCK - 1) / BLOCK;       /* 12 */
+constexpr int P_PER = PER_THREAD;
 
 /* bits of the per-slot byte over the halo-2 region */
 constexpr uint8_t S_HOLE = 1, S_EXCL = 2, S_IN = 8, S_A = 16, S_CAND = 32;
@@ -517,13 +527,20 @@ detect(Params p, int iter)
 	const double t_cr = p.threshold, t_nb = p.neighbor, t_min = fmin(t_cr, t_nb);
 	unsigned limit = iter == 0 ? (unsigned)p.ntiles : p.nlist[2 + iter];
 	const unsigned int *dlist = p.dirty_list + (size_t)(iter & 1) * p.ntiles;
+	/* thread 0 keeps the claim on the following tile in a register, so the
+	 * atomic's latency overlaps the current tile's loads */
+	unsigned next = 0;
+	if (!tid)
+		next = atomicAdd(&p.queue[iter], 1u);
 
 	for (;;) {
 		__syncthreads(); /* the previous tile's shared state is no longer read */
 		if (!tid) {
-			unsigned q = atomicAdd(&p.queue[iter], 1u);
+			unsigned q = next;
 			sh.tile = q < limit ? (iter == 0 ? (int)q : (int)dlist[q]) : -1;
 			sh.na = sh.nb = sh.unsafe = 0;
+			if (q < limit)
+				next = atomicAdd(&p.queue[iter], 1u);
 		}
 		__syncthreads();
 		const int tile = sh.tile;
@@ -531,7 +548,7 @@ detect(Params p, int iter)
 			break;
 		if (iter > 0 && !tid)
 			p.dirty[(size_t)(iter & 1) * p.ntiles + tile] = 0;
-		const int f = find_frame(p.frames, p.nf, tile);
+		const int f = p.tile_frame[tile];
 		const Frame fr = p.frames[f];
 		const TilePixels t = tile_of(fr, tile);
 		const int w = t.w, h = t.h, X0 = t.x0, Y0 = t.y0;
@@ -539,17 +556,81 @@ detect(Params p, int iter)
 		const bool narrow = w == 1;
 		if (!tid)
 			atomicAdd(&p.work[iter], (unsigned long long)(min(TILE_W, w - X0) * min(TILE_H, h - Y0)));
+		/* the halo-6 window lies inside the frame: no border mapping anywhere */
+		const bool inner = X0 >= HC && Y0 >= HC && X0 + TILE_W + HC <= w && Y0 + TILE_H + HC <= h;
 
-		/* 1. clean image with halo 6 under the median border mode */
-		for (int k = tid; k < CW * CH; k += BLOCK) {
-			int sx = k % CW, sy = k / CW;
-			int bx = border_index(X0 - HC + sx, w, mode), by = border_index(Y0 - HC + sy, h, mode);
-			sh.c[sy][sx] = bx < 0 || by < 0 ? 0.0 : clean[by * w + bx];
+		/* 1. issue every global load of the tile before the first use: the clean
+		 * image with halo 6 (median border mode), the flags with halo 2 and the
+		 * error map with halo 4. The slot arithmetic is recomputed per tile
+		 * (lane is opaque to the optimiser) rather than held in registers
+		 * across the persistent loop. */
+		int lane = tid;
+		asm volatile("" : "+v"(lane));
+		double cv[C_PER];
+#pragma unroll
+		for (int j = 0; j < C_PER; ++j) {
+			int k = lane + j * BLOCK;
+			cv[j] = 0.0;
+			if (k < CW * CH) {
+				int sx = k % CW, sy = k / CW;
+				int bx = X0 - HC + sx, by = Y0 - HC + sy;
+				if (!inner) {
+					bx = border_index(bx, w, mode);
+					by = border_index(by, h, mode);
+				}
+				if (bx >= 0 && by >= 0)
+					cv[j] = clean[by * w + bx];
+			}
 		}
-		for (int k = tid; k < PW * PH; k += BLOCK) {
-			int sx = k % PW, sy = k / PW, gx = X0 - HP + sx, gy = Y0 - HP + sy;
-			bool in = gx >= 0 && gx < w && gy >= 0 && gy < h;
-			sh.bits[sy][sx] = in ? (uint8_t)((p.flags[fr.off + gy * w + gx] & (HOLE | EXCL)) | S_IN) : 0;
+		uint8_t fv[P_PER];
+#pragma unroll
+		for (int j = 0; j < P_PER; ++j) {
+			int k = lane + j * BLOCK;
+			fv[j] = 0;
+			if (k < PW * PH) {
+				int gx = X0 - HP + k % PW, gy = Y0 - HP + k / PW;
+				if (gx >= 0 && gx < w && gy >= 0 && gy < h)
+					fv[j] = (uint8_t)((p.flags[fr.off + gy * w + gx] & (HOLE | EXCL)) | S_IN);
+			}
+		}
+		double ev[S_PER];
+		if (fr.error) {
+#pragma unroll
+			for (int j = 0; j < S_PER; ++j) {
+				int k = lane + j * BLOCK;
+				ev[j] = 1.0;
+				if (k < SW * SH) {
+					int gx = X0 - HS + k % SW, gy = Y0 - HS + k / SW;
+					bool in = gx >= 0 && gx < w && gy >= 0 && gy < h;
+					if (wrap && !in) {
+						gx = border_index(gx, w, mode);
+						gy = border_index(gy, h, mode);
+						in = true;
+					}
+					if (in)
+						ev[j] = fr.error[gy * w + gx];
+				}
+			}
+			/* staged in the significance array; each slot is later read and
+			 * overwritten by the thread that stored it */
+#pragma unroll
+			for (int j = 0; j < S_PER; ++j) {
+				int k = lane + j * BLOCK;
+				if (k < SW * SH)
+					sh.s[k / SW][k % SW] = ev[j];
+			}
+		}
+#pragma unroll
+		for (int j = 0; j < C_PER; ++j) {
+			int k = lane + j * BLOCK;
+			if (k < CW * CH)
+				sh.c[k / CW][k % CW] = cv[j];
+		}
+#pragma unroll
+		for (int j = 0; j < P_PER; ++j) {
+			int k = lane + j * BLOCK;
+			if (k < PW * PH)
+				sh.bits[k / PW][k % PW] = fv[j];
 		}
 		__syncthreads();
 
@@ -577,9 +658,7 @@ detect(Params p, int iter)
 					l = gx > 0 ? sh.c[cy][cx - 1] : c;
 					r = gx < w - 1 ? sh.c[cy][cx + 1] : c;
 				}
-				int bx = wrap ? border_index(gx, w, mode) : gx;
-				int by = wrap ? border_index(gy, h, mode) : gy;
-				double nz = fr.error[by * w + bx];
+				double nz = sh.s[sy][sx];
 				if (!(nz > 0) && !isnan(nz))
 					sh.unsafe = 1;
 				sh.s[sy][sx] = laplacian(c, u, d, l, r, exact, narrow) / (2 * nz);

@@ -60,7 +60,7 @@ class Frames:
 def run(frames, device, contrast, cr_threshold, neighbor_threshold, maxiter, border,
         gain=math.nan, readnoise=0.0, background=math.nan, frame_gain=None,
         frame_readnoise=None, record_donors=False, laplacian=_lib.LAPLACIAN_EXACT,
-        clean=None, mask=None):
+        clean=None, mask=None, profile=None):
     """Clean the frames in one call. Returns (clean, mask_u8, donors, passes) as
     flat device tensors (donors None unless recorded) and per-frame passes.
     A dict passed as ``profile`` receives the library's kernel timings."""

@@ -44,7 +44,8 @@ def test_arith_selftest(cuda):
                                           ctypes.c_void_p(q.data_ptr()), ctypes.c_void_p(r.data_ptr()),
                                           a.size, ctypes.c_void_p(stream))
     assert st == 0
-    assert_same_pixels(np_of(q), a / b)
+    with np.errstate(over="ignore"):
+        assert_same_pixels(np_of(q), a / b)
     assert_same_pixels(np_of(r), np.sqrt(a))
 
 
@@ -112,7 +113,12 @@ ORACLE_KEYS = dict(contrast="contrast", cr_threshold="cr_threshold",
 
 
 def compare(frames, laplacian="exact", **kw):
-    want = oracle.clean_frames(frames, laplacian_order=laplacian, **kw)
+    try:
+        want = oracle.clean_frames(frames, laplacian_order=laplacian, **kw)
+    except oracle.OracleError as e:
+        with pytest.raises(ValueError, match=str(e)):
+            gpu_clean(frames, laplacian=laplacian, **kw)
+        return None, None
     got = gpu_clean(frames, laplacian=laplacian, **kw)
     for f, (o, m, wo, wm) in enumerate(zip(got[0], got[1], want[0], want[1])):
         np.testing.assert_array_equal(m, wm, err_msg=f"mask of frame {f}")
