@@ -28,6 +28,9 @@
 
 using namespace cusmic;
 
+/* Persistent significance workgroups per CU (LDS 25 KB, 4 waves each). */
+constexpr int SIG_WGS_PER_CU = 4;
+
 namespace {
 
 void
@@ -232,7 +235,7 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	size_t n_counts = nf * (sizeof(Counts) / 4);
 	size_t n_todo = size_t(std::max(maxiter, 1)) * nf;
 	size_t n_queue = size_t(std::max(maxiter, 1));
-	size_t n_nlist = 2 + size_t(std::max(maxiter, 1));
+	size_t n_nlist = 2 + 4 * size_t(std::max(maxiter, 1));
 	size_t n_dirty = 2 * size_t(ntiles);
 	size_t words = n_counts + n_todo + n_queue + n_nlist + 1 + n_dirty;
 	size_t off_frames = 0;
@@ -241,8 +244,12 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	size_t off_work = align_up(off_dlist + n_dirty * 4, 256);
 	size_t off_tframe = align_up(off_work + n_queue * 8, 256);
 	size_t off_flags = align_up(off_tframe + size_t(ntiles) * 4, 256);
-	size_t off_list = align_up(off_flags + (maxiter ? total : 0), 256);
-	size_t bytes = off_list + (maxiter ? total * sizeof(unsigned long long) : 0);
+	size_t planes = maxiter ? total : 0;
+	size_t off_sprime = align_up(off_flags + planes * 4, 256);
+	size_t off_list = align_up(off_sprime + planes * 8, 256);
+	size_t off_list_b = align_up(off_list + planes * 8, 256);
+	size_t off_list_c = align_up(off_list_b + planes * 8, 256);
+	size_t bytes = off_list_c + planes * 8;
 	scratch mem(bytes, stream);
 	char *base = static_cast<char *>(mem.ptr);
 	hip_check(hipMemcpyAsync(base + off_frames, frames.data(), nf * sizeof(Frame),
@@ -272,8 +279,11 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	p.dirty_list = reinterpret_cast<unsigned int *>(base + off_dlist);
 	p.work = reinterpret_cast<unsigned long long *>(base + off_work);
 	p.tile_frame = reinterpret_cast<int *>(base + off_tframe);
-	p.flags = reinterpret_cast<uint8_t *>(base + off_flags);
+	p.flags = reinterpret_cast<uint32_t *>(base + off_flags);
+	p.sprime = reinterpret_cast<double *>(base + off_sprime);
 	p.list = reinterpret_cast<unsigned long long *>(base + off_list);
+	p.list_b = reinterpret_cast<unsigned long long *>(base + off_list_b);
+	p.list_c = reinterpret_cast<unsigned long long *>(base + off_list_c);
 
 	if (!maxiter) {
 		hipLaunchKernelGGL(copy_without_detection, dim3(ntiles), dim3(BLOCK), 0, stream, p);
@@ -283,6 +293,9 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	}
 
 	int cus = device_cus();
+	bool model = false; /* some frame uses the noise model instead of an error map */
+	for (size_t f = 0; f < nf; ++f)
+		model |= ims[f].error == nullptr;
 	timer tm(stream, profile != nullptr);
 	hipLaunchKernelGGL(zero_words, dim3(unsigned(std::min<size_t>((words + 255) / 256, 4096))),
 		dim3(256), 0, stream, w32, words);
@@ -294,10 +307,17 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	unsigned replace_grid = unsigned(cus) * 8;
 	hipLaunchKernelGGL(replace_pass, dim3(replace_grid), dim3(BLOCK), 0, stream, p, 0, 1);
 	tm.mark(CUSMIC_PROFILE_REPLACE_MS);
-	unsigned detect_grid = unsigned(std::min(ntiles, cus * 2));
+	unsigned sig_grid = unsigned(std::min(ntiles, cus * SIG_WGS_PER_CU));
+	unsigned list_grid = unsigned(cus) * 4;
 	for (int it = 0; it < maxiter; ++it) {
-		unsigned grid = it == 0 ? detect_grid : unsigned(cus * 2);
-		hipLaunchKernelGGL(detect, dim3(grid), dim3(BLOCK), 0, stream, p, it);
+		unsigned grid = it == 0 ? sig_grid : unsigned(cus * SIG_WGS_PER_CU);
+		if (model)
+			hipLaunchKernelGGL(significance<true>, dim3(grid), dim3(BLOCK), 0, stream, p, it);
+		else
+			hipLaunchKernelGGL(significance<false>, dim3(grid), dim3(BLOCK), 0, stream, p, it);
+		hipLaunchKernelGGL(fine, dim3(list_grid), dim3(BLOCK), 0, stream, p, it);
+		hipLaunchKernelGGL(grow_first, dim3(list_grid), dim3(BLOCK), 0, stream, p, it);
+		hipLaunchKernelGGL(grow_second, dim3(list_grid), dim3(BLOCK), 0, stream, p, it);
 		tm.mark(CUSMIC_PROFILE_DETECT_MS);
 		hipLaunchKernelGGL(replace_pass, dim3(replace_grid), dim3(BLOCK), 0, stream, p, it, 0);
 		tm.mark(CUSMIC_PROFILE_REPLACE_MS);

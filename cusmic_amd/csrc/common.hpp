@@ -14,14 +14,19 @@
 
 namespace cusmic {
 
-constexpr int TILE_W = 64;
+/* Detection tile: 56 x 32 outputs, so that the significance region with its
+ * 4-pixel halo is exactly one wave (64 lanes) wide. */
+constexpr int TILE_W = 56;
 constexpr int TILE_H = 32;
 constexpr int BLOCK = 256;
 
-/* Per-pixel state bits of the flags plane. */
-constexpr uint8_t HOLE = 1;    /* input pixel is nonfinite */
-constexpr uint8_t EXCL = 2;    /* hole or masked: never a seed or donor */
-constexpr uint8_t CRAY = 4;    /* flagged as cosmic ray (the output mask) */
+/* Per-pixel state word (u32) of the flags plane: bits 0-7 are flags, bits
+ * 16-31 hold the stamp of the last pass that computed S' at the pixel. */
+constexpr uint32_t HOLE = 1;    /* input pixel is nonfinite */
+constexpr uint32_t EXCL = 2;    /* hole or masked: never a seed or donor */
+constexpr uint32_t CRAY = 4;    /* flagged as cosmic ray (the output mask) */
+constexpr uint32_t GROW = 8;    /* on this pass's first-growth list */
+constexpr int STAMP_SHIFT = 16;
 
 constexpr double FINE_FLOOR = 0.01;   /* src/filters.cuh:21 */
 constexpr double SIGNAL_FLOOR = 1e-5; /* src/filters.cuh:22 */
@@ -51,14 +56,19 @@ struct Params {
 	int nf, ntiles, maxiter, mode, laplacian;
 	double contrast, threshold, neighbor, bg;  /* bg: scalar background or NaN */
 	double *clean;
-	uint8_t *flags;
+	uint32_t *flags;
 	uint8_t *mask;
+	double *sprime;                  /* S' where the stamp matches the pass */
+	unsigned long long *list_b;      /* seed candidates, then first-growth pixels */
+	unsigned long long *list_c;      /* candidates that passed the contrast test */
 	int32_t *donors;                 /* 2 per pixel or null */
 	unsigned long long *list;        /* hole list, then the flagged-pixel list */
 	Counts *counts;                  /* per frame */
 	int *todo;                       /* [maxiter][nf] new pixels per pass */
 	unsigned int *queue;             /* [maxiter] tile queue heads */
-	unsigned int *nlist;             /* [0] list length, [1] hole count, [2..] dirty counts */
+	unsigned int *nlist;             /* [0] list length, [1] hole count,
+					  * [2 + iter] dirty tiles, [2 + M + iter] seeds,
+					  * [2 + 2M + iter] candidates, [2 + 3M + iter] grown */
 	unsigned int *dirty;             /* [2][ntiles] dirty flags */
 	unsigned int *dirty_list;        /* [2][ntiles] dirty tile lists */
 	int *status;                     /* [0] error bits */
@@ -67,6 +77,12 @@ struct Params {
 };
 
 constexpr int ERR_NO_DONORS = 1;
+
+/* Offsets of the per-pass list counters in Params::nlist. */
+__device__ __forceinline__ int nl_dirty(const Params &p, int it) { return 2 + it; }
+__device__ __forceinline__ int nl_seed(const Params &p, int it) { return 2 + p.maxiter + it; }
+__device__ __forceinline__ int nl_cand(const Params &p, int it) { return 2 + 2 * p.maxiter + it; }
+__device__ __forceinline__ int nl_grow(const Params &p, int it) { return 2 + 3 * p.maxiter + it; }
 
 /* scipy.ndimage border mapping of coordinate x on an axis of n samples;
  * -1 means a CONSTANT-mode zero (src/filters.cuh:24-44). */

@@ -151,7 +151,7 @@ prologue(Params p)
 				c += 1 * (bg_map ? fr.background[i] : p.bg);
 			long long g = fr.off + i;
 			p.clean[g] = c;
-			p.flags[g] = (hole ? HOLE : 0) | (allow ? 0 : EXCL);
+			p.flags[g] = (hole ? HOLE : 0u) | (allow ? 0u : EXCL);
 			if (p.donors) {
 				p.donors[2 * g] = -1;
 				p.donors[2 * g + 1] = -1;
@@ -229,7 +229,7 @@ epilogue(Params p)
 			continue;
 		int i = y * t.w + x;
 		long long g = fr.off + i;
-		uint8_t fl = p.flags[g];
+		uint32_t fl = p.flags[g];
 		if (fl & HOLE)
 			p.clean[g] = fr.data[i];
 		else if (bg)
@@ -242,7 +242,7 @@ epilogue(Params p)
 /* Replacement (src/replace.cuh:112-153)                                     */
 
 __device__ __forceinline__ bool
-is_donor(const uint8_t *flags, const double *clean, int q, double &v)
+is_donor(const uint32_t *flags, const double *clean, int q, double &v)
 {
 	if (flags[q] & (EXCL | CRAY))
 		return false;
@@ -254,7 +254,7 @@ is_donor(const uint8_t *flags, const double *clean, int q, double &v)
  * search over the 64 key bits: each step counts donors below the candidate
  * prefix. Used only when the 5x5 window holds no donor. */
 __device__ unsigned long long
-window_select(const uint8_t *flags, const double *clean, int w, int x0, int x1, int y0, int y1,
+window_select(const uint32_t *flags, const double *clean, int w, int x0, int x1, int y0, int y1,
 	int rank)
 {
 	unsigned long long prefix = 0;
@@ -281,7 +281,7 @@ window_select(const uint8_t *flags, const double *clean, int w, int x0, int x1, 
 /* Index of the element of rank r in the (key, pixel index) order, given its
  * key: skip r - (#keys below) equal keys in raster order. */
 __device__ int
-rank_index(const uint8_t *flags, const double *clean, int w, int x0, int x1, int y0, int y1,
+rank_index(const uint32_t *flags, const double *clean, int w, int x0, int x1, int y0, int y1,
 	unsigned long long key, int r)
 {
 	int less = 0;
@@ -354,8 +354,10 @@ mark_dirty(const Params &p, const Frame &fr, int x, int y, int next)
 	for (int j = 0; j < cy; ++j)
 		for (int i = 0; i < cx; ++i) {
 			int tile = fr.tile0 + (ny < 0 ? j : ty[j]) * fr.tiles_x + (nx < 0 ? i : tx[i]);
-			if (!atomicExch(&flag[tile], 1u)) {
-				unsigned slot = atomicAdd(&p.nlist[2 + next], 1u);
+			/* most tiles are already marked by a neighbour: test before the
+			 * returning atomic */
+			if (!__atomic_load_n(&flag[tile], __ATOMIC_RELAXED) && !atomicExch(&flag[tile], 1u)) {
+				unsigned slot = atomicAdd(&p.nlist[nl_dirty(p, next)], 1u);
 				list[slot] = tile;
 			}
 		}
@@ -367,13 +369,13 @@ __device__ bool
 replace_pixel(const Params &p, const Frame &fr, int i, int32_t *donors)
 {
 	const double *clean = p.clean + fr.off;
-	const uint8_t *flags = p.flags + fr.off;
+	const uint32_t *flags = p.flags + fr.off;
 	int w = fr.w, h = fr.h, x = i % w, y = i / w;
 	/* flags and values of the 24 neighbours are loaded together (one memory
 	 * round trip); a value read at a flagged neighbour, which another lane may
 	 * be rewriting, is discarded */
 	double v[24];
-	uint8_t fl[24];
+	uint32_t fl[24];
 	int n = 0, neg0 = 0;
 #pragma unroll
 	for (int k = 0; k < 24; ++k) {
@@ -488,57 +490,123 @@ replace_pass(Params p, int iter, int fill)
 
 /* ------------------------------------------------------------------------ */
 /* Detection (src/filters.cuh:86-193, src/cleaner.cuh:22-37,102-126)         */
+/*
+ * One detection pass is four launches:
+ *   significance  dense, per 56 x 32 tile: S = L / (2 N) with a 4-pixel halo
+ *                 in LDS; S' = S - med5(S) only where S > min(t, t_n) (S' <= S
+ *                 because S >= 0); S' is written to a sparse plane stamped
+ *                 with the pass; seeds (S' > t, not excluded) go to a list
+ *   fine          per seed: F = max((m3 - med7(m3)) / N, 0.01), m3 = med3(C);
+ *                 S' / F > contrast puts the seed on the candidate list
+ *   grow_first    per candidate: neighbours with S' > t join the growth list
+ *   grow_second   per grown pixel: neighbours with S' > t_n are flagged; a
+ *                 pixel flagged for the first time is new (accumulate)
+ * Every list entry is a correct member of its set, and a pixel outside the
+ * processed tiles cannot be new, so the sparse passes give the reference's
+ * dense result exactly.
+ */
 
-/* LDS regions, in slots relative to the tile origin: the clean image with a
- * halo of 6, the significance with a halo of 4, per-pixel bits and growth
- * with halos of 2 and 1. */
-constexpr int HC = 6, HS = 4, HP = 2;
-constexpr int CW = TILE_W + 2 * HC, CH = TILE_H + 2 * HC;  /* 76 x 44 */
-constexpr int SW = TILE_W + 2 * HS, SH = TILE_H + 2 * HS;  /* 72 x 40 */
-constexpr int PW = TILE_W + 2 * HP, PH = TILE_H + 2 * HP;  /* 68 x 36 */
-constexpr int GW = TILE_W + 2, GH = TILE_H + 2;            /* 66 x 34 */
-constexpr int PER_THREAD = (PW * PH + BLOCK - 1) / BLOCK;  /* 10 */
-constexpr int C_PER = (CW * CH + BLOCK - 1) / BLOCK;       /* 14 */
-constexpr int S_PER = (SW * SH + BLOCK - 1) / BLOCK;       /* 12 */
-constexpr int P_PER = PER_THREAD;
+constexpr int HS = 4, HP = 2;
+constexpr int SW = TILE_W + 2 * HS, SH = TILE_H + 2 * HS;  /* 64 x 40 */
+constexpr int PW = TILE_W + 2 * HP, PH = TILE_H + 2 * HP;  /* 60 x 36 */
+constexpr int ROWS_PER_WAVE = SH / 4;                      /* 10 */
+static_assert(SW == 64, "the significance region is one wave wide");
+static_assert(SH % 4 == 0, "four waves share the significance rows");
 
-/* bits of the per-slot byte over the halo-2 region */
-constexpr uint8_t S_HOLE = 1, S_EXCL = 2, S_IN = 8, S_A = 16, S_CAND = 32;
+__device__ __forceinline__ unsigned
+pass_stamp(int iter)
+{
+	return unsigned(iter % 65535) + 1;
+}
 
-struct DetectShared {
-	double c[CH][CW];
+/* Laplacian sample of the clean image at (x, y), at most one step outside the
+ * frame for in-frame centres: mirror/reflect/nearest repeat the edge pixel,
+ * wrap wraps, constant reads 0 (src/filters.cuh:72-105). Far-outside
+ * coordinates (halo slots that are later overwritten) stay in bounds. */
+__device__ __forceinline__ double
+lap_sample(const double *c, int x, int y, int w, int h, int mode)
+{
+	if (x >= 0 && x < w && y >= 0 && y < h)
+		return c[y * w + x];
+	if (mode == CUSMIC_CONSTANT)
+		return 0.0;
+	if (mode == CUSMIC_WRAP) {
+		x = border_index(x, w, mode);
+		y = border_index(y, h, mode);
+	} else {
+		x = min(max(x, 0), w - 1);
+		y = min(max(y, 0), h - 1);
+	}
+	return c[y * w + x];
+}
+
+/* Median-filter sample under the border mode (scipy semantics). */
+__device__ __forceinline__ double
+med_sample(const double *c, int x, int y, int w, int h, int mode)
+{
+	int bx = border_index(x, w, mode), by = border_index(y, h, mode);
+	return bx < 0 || by < 0 ? 0.0 : c[by * w + bx];
+}
+
+/* In-bounds index for a map read at a halo slot. */
+__device__ __forceinline__ int
+map_index(int x, int y, int w, int h, int mode)
+{
+	if (mode == CUSMIC_WRAP) {
+		x = border_index(x, w, mode);
+		y = border_index(y, h, mode);
+	} else {
+		x = min(max(x, 0), w - 1);
+		y = min(max(y, 0), h - 1);
+	}
+	return y * w + x;
+}
+
+__device__ __forceinline__ void
+list_push(bool pred, unsigned long long item, unsigned long long *list, unsigned int *count)
+{
+	unsigned long long m = __ballot(pred);
+	if (!m)
+		return;
+	int lane = threadIdx.x & 63, leader = __ffsll((long long)m) - 1;
+	unsigned base = 0;
+	if (lane == leader)
+		base = atomicAdd(count, (unsigned)__popcll(m));
+	base = __shfl(base, leader, 64);
+	if (pred)
+		list[base + __popcll(m & ((1ull << lane) - 1))] = item;
+}
+
+struct SigShared {
 	double s[SH][SW];
-	uint8_t bits[PH][PW];
-	uint8_t g1[GH][GW];
 	uint16_t alist[PW * PH];
-	uint16_t blist[PW * PH];
-	int na, nb, tile, unsafe;
-	int red[4];
+	int na, tile, unsafe;
 };
 
-__global__ __launch_bounds__(BLOCK, 2) void
-detect(Params p, int iter)
+template <bool MODEL>
+__global__ __launch_bounds__(BLOCK) void
+significance(Params p, int iter)
 {
-	__shared__ DetectShared sh;
-	const int tid = threadIdx.x;
+	__shared__ SigShared sh;
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 	const int mode = p.mode;
 	const bool wrap = mode == CUSMIC_WRAP;
 	const bool exact = p.laplacian == 0;
-	const double t_cr = p.threshold, t_nb = p.neighbor, t_min = fmin(t_cr, t_nb);
-	unsigned limit = iter == 0 ? (unsigned)p.ntiles : p.nlist[2 + iter];
+	const double t_cr = p.threshold, t_min = fmin(p.threshold, p.neighbor);
+	const unsigned stamp = pass_stamp(iter);
+	const unsigned limit = iter == 0 ? (unsigned)p.ntiles : p.nlist[nl_dirty(p, iter)];
 	const unsigned int *dlist = p.dirty_list + (size_t)(iter & 1) * p.ntiles;
-	/* thread 0 keeps the claim on the following tile in a register, so the
-	 * atomic's latency overlaps the current tile's loads */
+	unsigned int *seeds = &p.nlist[nl_seed(p, iter)];
 	unsigned next = 0;
 	if (!tid)
 		next = atomicAdd(&p.queue[iter], 1u);
 
 	for (;;) {
-		__syncthreads(); /* the previous tile's shared state is no longer read */
+		__syncthreads();
 		if (!tid) {
 			unsigned q = next;
 			sh.tile = q < limit ? (iter == 0 ? (int)q : (int)dlist[q]) : -1;
-			sh.na = sh.nb = sh.unsafe = 0;
+			sh.na = sh.unsafe = 0;
 			if (q < limit)
 				next = atomicAdd(&p.queue[iter], 1u);
 		}
@@ -556,347 +624,302 @@ detect(Params p, int iter)
 		const bool narrow = w == 1;
 		if (!tid)
 			atomicAdd(&p.work[iter], (unsigned long long)(min(TILE_W, w - X0) * min(TILE_H, h - Y0)));
-		/* the halo-6 window lies inside the frame: no border mapping anywhere */
-		const bool inner = X0 >= HC && Y0 >= HC && X0 + TILE_W + HC <= w && Y0 + TILE_H + HC <= h;
 
-		/* 1. issue every global load of the tile before the first use: the clean
-		 * image with halo 6 (median border mode), the flags with halo 2 and the
-		 * error map with halo 4. The slot arithmetic is recomputed per tile
-		 * (lane is opaque to the optimiser) rather than held in registers
-		 * across the persistent loop. */
-		int lane = tid;
-		asm volatile("" : "+v"(lane));
-		double cv[C_PER];
-#pragma unroll
-		for (int j = 0; j < C_PER; ++j) {
-			int k = lane + j * BLOCK;
-			cv[j] = 0.0;
-			if (k < CW * CH) {
-				int sx = k % CW, sy = k / CW;
-				int bx = X0 - HC + sx, by = Y0 - HC + sy;
-				if (!inner) {
-					bx = border_index(bx, w, mode);
-					by = border_index(by, h, mode);
-				}
-				if (bx >= 0 && by >= 0)
-					cv[j] = clean[by * w + bx];
-			}
-		}
-		uint8_t fv[P_PER];
-#pragma unroll
-		for (int j = 0; j < P_PER; ++j) {
-			int k = lane + j * BLOCK;
-			fv[j] = 0;
-			if (k < PW * PH) {
-				int gx = X0 - HP + k % PW, gy = Y0 - HP + k / PW;
-				if (gx >= 0 && gx < w && gy >= 0 && gy < h)
-					fv[j] = (uint8_t)((p.flags[fr.off + gy * w + gx] & (HOLE | EXCL)) | S_IN);
-			}
-		}
-		double ev[S_PER];
-		if (fr.error) {
-#pragma unroll
-			for (int j = 0; j < S_PER; ++j) {
-				int k = lane + j * BLOCK;
-				ev[j] = 1.0;
-				if (k < SW * SH) {
-					int gx = X0 - HS + k % SW, gy = Y0 - HS + k / SW;
-					bool in = gx >= 0 && gx < w && gy >= 0 && gy < h;
-					if (wrap && !in) {
-						gx = border_index(gx, w, mode);
-						gy = border_index(gy, h, mode);
-						in = true;
-					}
-					if (in)
-						ev[j] = fr.error[gy * w + gx];
-				}
-			}
-			/* staged in the significance array; each slot is later read and
-			 * overwritten by the thread that stored it */
-#pragma unroll
-			for (int j = 0; j < S_PER; ++j) {
-				int k = lane + j * BLOCK;
-				if (k < SW * SH)
-					sh.s[k / SW][k % SW] = ev[j];
-			}
-		}
-#pragma unroll
-		for (int j = 0; j < C_PER; ++j) {
-			int k = lane + j * BLOCK;
-			if (k < CW * CH)
-				sh.c[k / CW][k % CW] = cv[j];
-		}
-#pragma unroll
-		for (int j = 0; j < P_PER; ++j) {
-			int k = lane + j * BLOCK;
-			if (k < PW * PH)
-				sh.bits[k / PW][k % PW] = fv[j];
-		}
-		__syncthreads();
-
-		/* 2. significance S = L / (2 N) over the halo-4 region: in-frame slots,
-		 * or every slot under WRAP, whose stencils commute with the periodic
-		 * extension. */
-		if (fr.error) {
-			for (int k = tid; k < SW * SH; k += BLOCK) {
-				int sx = k % SW, sy = k / SW, gx = X0 - HS + sx, gy = Y0 - HS + sy;
+		/* 1. S over the 64 x 40 region: lane = column, each wave streams ten
+		 * rows. Slots outside the frame are computed from in-bounds samples
+		 * and replaced in step 2 unless the mode is WRAP, under which every
+		 * stencil commutes with the periodic extension. */
+		const int gx = X0 - HS + lane;
+		const int r0 = wave * ROWS_PER_WAVE;
+		bool bad = false;
+		if (!MODEL || fr.error) {
+			double up = lap_sample(clean, gx, Y0 - HS + r0 - 1, w, h, mode);
+			double cur = lap_sample(clean, gx, Y0 - HS + r0, w, h, mode);
+#pragma unroll 5
+			for (int r = 0; r < ROWS_PER_WAVE; ++r) {
+				int gy = Y0 - HS + r0 + r;
+				double dn = lap_sample(clean, gx, gy + 1, w, h, mode);
+				double l = lap_sample(clean, gx - 1, gy, w, h, mode);
+				double rt = lap_sample(clean, gx + 1, gy, w, h, mode);
+				double nz = fr.error[map_index(gx, gy, w, h, mode)];
 				bool in = gx >= 0 && gx < w && gy >= 0 && gy < h;
-				if (!in && !wrap)
-					continue;
-				int cx = sx + (HC - HS), cy = sy + (HC - HS);
-				double c = sh.c[cy][cx];
-				double u, d, l, r;
-				if (wrap || mode == CUSMIC_CONSTANT) {
-					u = sh.c[cy - 1][cx];
-					d = sh.c[cy + 1][cx];
-					l = sh.c[cy][cx - 1];
-					r = sh.c[cy][cx + 1];
-				} else {
-					/* mirror/reflect/nearest: a replicated edge samples itself */
-					u = gy > 0 ? sh.c[cy - 1][cx] : c;
-					d = gy < h - 1 ? sh.c[cy + 1][cx] : c;
-					l = gx > 0 ? sh.c[cy][cx - 1] : c;
-					r = gx < w - 1 ? sh.c[cy][cx + 1] : c;
-				}
-				double nz = sh.s[sy][sx];
-				if (!(nz > 0) && !isnan(nz))
-					sh.unsafe = 1;
-				sh.s[sy][sx] = laplacian(c, u, d, l, r, exact, narrow) / (2 * nz);
+				bad |= (in || wrap) && !(nz > 0) && !isnan(nz);
+				sh.s[r0 + r][lane] = laplacian(cur, up, dn, l, rt, exact, narrow) / (2 * nz);
+				up = cur;
+				cur = dn;
 			}
 		} else {
-			/* noise model: N from the 5x5 median of C, 4 rows per item */
-			constexpr int STRIPS = SH / 4;
-			for (int k = tid; k < SW * STRIPS; k += BLOCK) {
-				int sx = k % SW, sy0 = (k / SW) * 4;
+			/* noise model: N from strip-shared 5x5 medians, four rows at a time */
+#pragma unroll 1
+			for (int g = 0; g < ROWS_PER_WAVE; g += 4) {
+				int gy0 = Y0 - HS + r0 + g;
 				double rows[8 * 5], med[4];
 #pragma unroll
 				for (int j = 0; j < 8; ++j)
 #pragma unroll
 					for (int c = 0; c < 5; ++c)
-						rows[j * 5 + c] = sh.c[sy0 + j][sx + c];
+						rows[j * 5 + c] = med_sample(clean, gx - 2 + c, gy0 - 2 + j, w, h, mode);
 				strip_median5x4(rows, med);
 #pragma unroll
 				for (int o = 0; o < 4; ++o) {
-					int sy = sy0 + o, gx = X0 - HS + sx, gy = Y0 - HS + sy;
+					if (g + o >= ROWS_PER_WAVE)
+						break;
+					int gy = gy0 + o;
+					double cur = lap_sample(clean, gx, gy, w, h, mode);
+					double up = lap_sample(clean, gx, gy - 1, w, h, mode);
+					double dn = lap_sample(clean, gx, gy + 1, w, h, mode);
+					double l = lap_sample(clean, gx - 1, gy, w, h, mode);
+					double rt = lap_sample(clean, gx + 1, gy, w, h, mode);
+					int mi = map_index(gx, gy, w, h, mode);
+					double gg = fr.gain ? fr.gain[mi] : fr.g;
+					double rn = fr.readnoise ? fr.readnoise[mi] : fr.rn;
+					double nz = noise_model(med[o], gg, rn);
 					bool in = gx >= 0 && gx < w && gy >= 0 && gy < h;
-					if (!in && !wrap)
-						continue;
-					int cx = sx + (HC - HS), cy = sy + (HC - HS);
-					double c = sh.c[cy][cx];
-					double u, d, l, r;
-					if (wrap || mode == CUSMIC_CONSTANT) {
-						u = sh.c[cy - 1][cx];
-						d = sh.c[cy + 1][cx];
-						l = sh.c[cy][cx - 1];
-						r = sh.c[cy][cx + 1];
-					} else {
-						u = gy > 0 ? sh.c[cy - 1][cx] : c;
-						d = gy < h - 1 ? sh.c[cy + 1][cx] : c;
-						l = gx > 0 ? sh.c[cy][cx - 1] : c;
-						r = gx < w - 1 ? sh.c[cy][cx + 1] : c;
-					}
-					int bx = border_index(gx, w, mode), by = border_index(gy, h, mode);
-					double g = fr.gain ? fr.gain[by * w + bx] : fr.g;
-					double rn = fr.readnoise ? fr.readnoise[by * w + bx] : fr.rn;
-					double nz = noise_model(med[o], g, rn);
-					if (!(nz > 0) && !isnan(nz))
-						sh.unsafe = 1;
-					sh.s[sy][sx] = laplacian(c, u, d, l, r, exact, narrow) / (2 * nz);
+					bad |= (in || wrap) && !(nz > 0) && !isnan(nz);
+					sh.s[r0 + g + o][lane] = laplacian(cur, up, dn, l, rt, exact, narrow) / (2 * nz);
 				}
 			}
 		}
+		if (bad)
+			sh.unsafe = 1;
 		__syncthreads();
 
-		/* 3. out-of-frame significance slots read by the 5x5 median */
+		/* 2. out-of-frame slots read by the 5x5 median: the mapped in-frame
+		 * slot (median border mode), or 0 for CONSTANT */
 		const bool edge = X0 - HS < 0 || Y0 - HS < 0 || X0 + TILE_W + HS > w || Y0 + TILE_H + HS > h;
 		if (edge && !wrap) {
 			for (int k = tid; k < SW * SH; k += BLOCK) {
-				int sx = k % SW, sy = k / SW, gx = X0 - HS + sx, gy = Y0 - HS + sy;
-				if (gx >= 0 && gx < w && gy >= 0 && gy < h)
-					continue;
-				if (gx < -2 || gx > w + 1 || gy < -2 || gy > h + 1)
+				int sx = k % SW, sy = k / SW, x = X0 - HS + sx, y = Y0 - HS + sy;
+				if ((x >= 0 && x < w && y >= 0 && y < h) || x < -2 || x > w + 1 || y < -2 || y > h + 1)
 					continue;
 				double v = 0.0;
-				if (mode != CUSMIC_CONSTANT) {
-					int bx = border_index(gx, w, mode), by = border_index(gy, h, mode);
-					v = sh.s[by - (Y0 - HS)][bx - (X0 - HS)];
-				}
+				if (mode != CUSMIC_CONSTANT)
+					v = sh.s[border_index(y, h, mode) - (Y0 - HS)][border_index(x, w, mode) - (X0 - HS)];
 				sh.s[sy][sx] = v;
 			}
 			__syncthreads();
 		}
 
-		/* 4. screen: S' = S - med5(S) <= S because S >= 0 and so med5(S) >= 0;
-		 * where S <= min(t, t_n) no threshold test can pass. An invalid noise
-		 * value (<= 0) disables the screen for the tile. */
+		/* 3. screen: S' = S - med5(S) <= S, so only S > min(t, t_n) can pass a
+		 * threshold; a tile with an invalid noise value (<= 0) keeps every
+		 * pixel */
 		const bool unsafe = sh.unsafe != 0;
 		for (int k = tid; k < PW * PH; k += BLOCK) {
-			int sx = k % PW, sy = k / PW;
-			uint8_t b = sh.bits[sy][sx];
-			bool a = (b & S_IN) && !(b & S_HOLE) &&
+			int sx = k % PW, sy = k / PW, x = X0 - HP + sx, y = Y0 - HP + sy;
+			bool a = x >= 0 && x < w && y >= 0 && y < h &&
 				 (unsafe || sh.s[sy + (HS - HP)][sx + (HS - HP)] > t_min);
 			lds_append(a, (uint16_t)k, sh.alist, &sh.na);
 		}
 		__syncthreads();
 
-		/* 5. S' at screened pixels; written back after every read is done */
+		/* 4. S' at the screened pixels (holes excluded: S' = 0 there), stamped
+		 * into the sparse plane; seeds to the list */
 		const int na = sh.na;
-		double sp[PER_THREAD];
-#pragma unroll
-		for (int j = 0; j < PER_THREAD; ++j) {
-			int idx = tid + j * BLOCK;
-			sp[j] = 0.0;
+		for (int base = 0; base < na; base += BLOCK) {
+			int idx = base + tid;
+			bool seed = false;
+			int i = 0;
 			if (idx < na) {
-				int pos = sh.alist[idx], sx = pos % PW + (HS - HP), sy = pos / PW + (HS - HP);
-				double v[25], m;
+				int pos = sh.alist[idx], sx = pos % PW, sy = pos / PW;
+				int x = X0 - HP + sx, y = Y0 - HP + sy;
+				i = y * w + x;
+				uint32_t fl = p.flags[fr.off + i];
+				if (!(fl & HOLE)) {
+					int cx = sx + (HS - HP), cy = sy + (HS - HP);
+					double v[25], m;
 #pragma unroll
-				for (int dy = 0; dy < 5; ++dy)
+					for (int dy = 0; dy < 5; ++dy)
 #pragma unroll
-					for (int dx = 0; dx < 5; ++dx)
-						v[dy * 5 + dx] = sh.s[sy - 2 + dy][sx - 2 + dx];
-				median25(v, &m);
-				sp[j] = sh.s[sy][sx] - m;
-			}
-		}
-		__syncthreads();
-#pragma unroll
-		for (int j = 0; j < PER_THREAD; ++j) {
-			int idx = tid + j * BLOCK;
-			bool b = false;
-			uint16_t pos = 0;
-			if (idx < na) {
-				pos = sh.alist[idx];
-				int sx = pos % PW, sy = pos / PW;
-				sh.s[sy + (HS - HP)][sx + (HS - HP)] = sp[j];
-				sh.bits[sy][sx] |= S_A;
-				b = !(sh.bits[sy][sx] & S_EXCL) && sp[j] > t_cr;
-			}
-			if (j * BLOCK < na)
-				lds_append(b, pos, sh.blist, &sh.nb);
-		}
-		__syncthreads();
-		const int nb = sh.nb;
-		if (!nb)
-			continue; /* no seed candidates near this tile: nothing new */
-
-		/* 6. fine structure at seed candidates: F = max((m3 - med7(m3)) / N,
-		 * 0.01) with m3 = med3(C); cand = S' / F > contrast */
-		for (int idx = tid; idx < nb; idx += BLOCK) {
-			int pos = sh.blist[idx], px = pos % PW, py = pos / PW;
-			int gx = X0 - HP + px, gy = Y0 - HP + py;
-			double m3[49];
-#pragma unroll
-			for (int dy = -3; dy <= 3; ++dy)
-#pragma unroll
-				for (int dx = -3; dx <= 3; ++dx) {
-					int qx = gx + dx, qy = gy + dy;
-					bool zero = false;
-					if (!wrap && (qx < 0 || qx >= w || qy < 0 || qy >= h)) {
-						if (mode == CUSMIC_CONSTANT)
-							zero = true;
-						else {
-							qx = border_index(qx, w, mode);
-							qy = border_index(qy, h, mode);
-						}
-					}
-					double val = 0.0;
-					if (!zero) {
-						int cx = qx - X0 + HC, cy = qy - Y0 + HC;
-						double v9[9];
-#pragma unroll
-						for (int j = 0; j < 3; ++j)
-#pragma unroll
-							for (int i = 0; i < 3; ++i)
-								v9[j * 3 + i] = sh.c[cy - 1 + j][cx - 1 + i];
-						median9(v9, &val);
-					}
-					m3[(dy + 3) * 7 + dx + 3] = val;
-				}
-			double med7;
-			median49(m3, &med7);
-			double nz;
-			if (fr.error) {
-				nz = fr.error[gy * w + gx];
-			} else {
-				int cx = gx - X0 + HC, cy = gy - Y0 + HC;
-				double v25[25], m5;
-#pragma unroll
-				for (int j = 0; j < 5; ++j)
-#pragma unroll
-					for (int i = 0; i < 5; ++i)
-						v25[j * 5 + i] = sh.c[cy - 2 + j][cx - 2 + i];
-				median25(v25, &m5);
-				double g = fr.gain ? fr.gain[gy * w + gx] : fr.g;
-				double rn = fr.readnoise ? fr.readnoise[gy * w + gx] : fr.rn;
-				nz = noise_model(m5, g, rn);
-			}
-			double fine = (m3[24] - med7) / nz;
-			fine = fine < FINE_FLOOR ? FINE_FLOOR : fine;
-			double sig = sh.s[py + (HS - HP)][px + (HS - HP)];
-			if (sig / fine > p.contrast)
-				sh.bits[py][px] |= S_CAND;
-		}
-		__syncthreads();
-
-		/* 7. first growth at the cosmic-ray threshold over the halo-1 region */
-		for (int k = tid; k < GW * GH; k += BLOCK) {
-			int gxs = k % GW, gys = k / GW, px = gxs + 1, py = gys + 1;
-			uint8_t b = sh.bits[py][px];
-			bool g = false;
-			if ((b & S_A) && sh.s[py + (HS - HP)][px + (HS - HP)] > t_cr) {
-#pragma unroll
-				for (int dy = -1; dy <= 1; ++dy)
-#pragma unroll
-					for (int dx = -1; dx <= 1; ++dx)
-						g |= (sh.bits[py + dy][px + dx] & S_CAND) != 0;
-			}
-			sh.g1[gys][gxs] = g;
-		}
-		__syncthreads();
-
-		/* 8. second growth at the neighbour threshold over the tile, then
-		 * accumulate: new = grown & !flagged (src/cleaner.cuh:22-37) */
-		int fresh = 0, lost = 0;
-		for (int k = tid; k < TILE_W * TILE_H; k += BLOCK) {
-			int tx = k % TILE_W, ty = k / TILE_W, px = tx + HP, py = ty + HP;
-			uint8_t b = sh.bits[py][px];
-			bool g = false;
-			if ((b & S_A) && sh.s[py + (HS - HP)][px + (HS - HP)] > t_nb) {
-#pragma unroll
-				for (int dy = -1; dy <= 1; ++dy)
-#pragma unroll
-					for (int dx = -1; dx <= 1; ++dx)
-						g |= sh.g1[ty + 1 + dy][tx + 1 + dx] != 0;
-			}
-			bool isnew = false;
-			int i = (Y0 + ty) * w + X0 + tx;
-			if (g) {
-				uint8_t fl = p.flags[fr.off + i];
-				if (!(fl & CRAY)) {
-					isnew = true;
-					p.flags[fr.off + i] = fl | CRAY;
-					fresh += 1;
-					lost += !(fl & EXCL) && isfinite(sh.c[ty + HC][tx + HC]);
+						for (int dx = 0; dx < 5; ++dx)
+							v[dy * 5 + dx] = sh.s[cy - 2 + dy][cx - 2 + dx];
+					median25(v, &m);
+					double sp = sh.s[cy][cx] - m;
+					p.sprime[fr.off + i] = sp;
+					reinterpret_cast<uint16_t *>(p.flags + fr.off + i)[1] = (uint16_t)stamp;
+					seed = !(fl & EXCL) && sp > t_cr;
 				}
 			}
-			unsigned long long m = __ballot(isnew);
-			if (m) {
-				int lane = tid & 63, leader = __ffsll((long long)m) - 1;
-				unsigned base = 0;
-				if (lane == leader)
-					base = atomicAdd(&p.nlist[0], (unsigned)__popcll(m));
-				base = __shfl(base, leader, 64);
-				if (isnew)
-					p.list[base + __popcll(m & ((1ull << lane) - 1))] = list_entry(f, i);
+			list_push(seed, list_entry(f, i), p.list_b, seeds);
+		}
+	}
+}
+
+/* F at a seed from the 9 x 9 clean-image block around it (median border
+ * mode): 49 med3 values, their med7, and N (src/filters.cuh:148-162). */
+__device__ double
+fine_structure(const Params &p, const Frame &fr, int x, int y)
+{
+	const double *clean = p.clean + fr.off;
+	const int w = fr.w, h = fr.h, mode = p.mode;
+	double m3[49];
+	const bool inside = x >= 3 && x + 3 < w && y >= 3 && y + 3 < h;
+	if (inside || mode != CUSMIC_NEAREST) {
+		/* in the extended image of a symmetric or periodic mode, med3 at an
+		 * outside position equals med3 at its mapped position; CONSTANT
+		 * positions outside the frame are 0. Rows of the 9 x 9 block roll
+		 * through three registers sets. */
+		double a[9], b[9], c[9];
+#pragma unroll
+		for (int i = 0; i < 9; ++i) {
+			a[i] = med_sample(clean, x - 4 + i, y - 4, w, h, mode);
+			b[i] = med_sample(clean, x - 4 + i, y - 3, w, h, mode);
+		}
+#pragma unroll
+		for (int j = 0; j < 7; ++j) {
+#pragma unroll
+			for (int i = 0; i < 9; ++i)
+				c[i] = med_sample(clean, x - 4 + i, y - 2 + j, w, h, mode);
+			/* keep the loads of later rows from being hoisted here */
+			__builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+			for (int i = 0; i < 7; ++i) {
+				double v9[9] = {a[i], a[i + 1], a[i + 2], b[i], b[i + 1], b[i + 2],
+						c[i], c[i + 1], c[i + 2]};
+				double m;
+				median9(v9, &m);
+				int qx = x - 3 + i, qy = y - 3 + j;
+				bool out = qx < 0 || qx >= w || qy < 0 || qy >= h;
+				m3[j * 7 + i] = mode == CUSMIC_CONSTANT && out ? 0.0 : m;
+			}
+#pragma unroll
+			for (int i = 0; i < 9; ++i) {
+				a[i] = b[i];
+				b[i] = c[i];
 			}
 		}
-		fresh = block_sum(fresh, sh.red);
-		lost = block_sum(lost, sh.red);
-		if (!tid) {
-			if (fresh)
-				atomicAdd(&p.todo[iter * p.nf + f], fresh);
-			if (lost)
-				atomicSub(&p.counts[f].donors, lost);
+	} else {
+		/* NEAREST near the edge: med3 of the clamped position */
+		for (int j = 0; j < 7; ++j)
+			for (int i = 0; i < 7; ++i) {
+				int qx = min(max(x - 3 + i, 0), w - 1), qy = min(max(y - 3 + j, 0), h - 1);
+				double v9[9], m;
+				for (int a = 0; a < 3; ++a)
+					for (int c = 0; c < 3; ++c)
+						v9[a * 3 + c] = med_sample(clean, qx - 1 + c, qy - 1 + a, w, h, mode);
+				median9(v9, &m);
+				m3[j * 7 + i] = m;
+			}
+	}
+	double med7;
+	median49(m3, &med7);
+	double nz;
+	if (fr.error) {
+		nz = fr.error[y * w + x];
+	} else {
+		double v25[25], m5;
+		for (int j = 0; j < 5; ++j)
+			for (int i = 0; i < 5; ++i)
+				v25[j * 5 + i] = med_sample(clean, x - 2 + i, y - 2 + j, w, h, mode);
+		median25(v25, &m5);
+		double g = fr.gain ? fr.gain[y * w + x] : fr.g;
+		double rn = fr.readnoise ? fr.readnoise[y * w + x] : fr.rn;
+		nz = noise_model(m5, g, rn);
+	}
+	double fine = (m3[24] - med7) / nz;
+	return fine < FINE_FLOOR ? FINE_FLOOR : fine;
+}
+
+__global__ __launch_bounds__(BLOCK) void
+fine(Params p, int iter)
+{
+	const unsigned n = p.nlist[nl_seed(p, iter)];
+	unsigned int *count = &p.nlist[nl_cand(p, iter)];
+	for (unsigned base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
+		unsigned e = base + threadIdx.x;
+		bool cand = false;
+		unsigned long long entry = 0;
+		if (e < n) {
+			entry = p.list_b[e];
+			int f = (int)(entry >> 32), i = (int)(entry & 0xffffffffu);
+			const Frame fr = p.frames[f];
+			double sp = p.sprime[fr.off + i];
+			cand = sp / fine_structure(p, fr, i % fr.w, i / fr.w) > p.contrast;
+		}
+		list_push(cand, entry, p.list_c, count);
+	}
+}
+
+/* S' at a pixel if this pass computed it there, else -inf (cannot pass). */
+__device__ __forceinline__ double
+sprime_at(const Params &p, long long g, unsigned stamp)
+{
+	return (p.flags[g] >> STAMP_SHIFT) == stamp ? p.sprime[g] : -INFINITY;
+}
+
+/* First growth (src/filters.cuh:177-193 at t): every pixel next to a
+ * candidate with S' > t joins the growth list once. */
+__global__ __launch_bounds__(BLOCK) void
+grow_first(Params p, int iter)
+{
+	const unsigned n = p.nlist[nl_cand(p, iter)];
+	const unsigned stamp = pass_stamp(iter);
+	unsigned int *count = &p.nlist[nl_grow(p, iter)];
+	for (unsigned base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
+		unsigned e = base + threadIdx.x;
+		int f = 0, x = 0, y = 0, w = 1, h = 1;
+		long long off = 0;
+		if (e < n) {
+			unsigned long long entry = p.list_c[e];
+			f = (int)(entry >> 32);
+			int i = (int)(entry & 0xffffffffu);
+			const Frame fr = p.frames[f];
+			w = fr.w;
+			h = fr.h;
+			off = fr.off;
+			x = i % w;
+			y = i / w;
+		}
+		for (int k = 0; k < 9; ++k) {
+			int qx = x + k % 3 - 1, qy = y + k / 3 - 1;
+			bool join = false;
+			int q = qy * w + qx;
+			if (e < n && qx >= 0 && qx < w && qy >= 0 && qy < h && sprime_at(p, off + q, stamp) > p.threshold)
+				join = !(atomicOr(&p.flags[off + q], GROW) & GROW);
+			list_push(join, list_entry(f, q), p.list_b, count);
+		}
+	}
+}
+
+/* Second growth at t_n and accumulation (src/cleaner.cuh:22-37): a pixel next
+ * to a grown pixel with S' > t_n is flagged; if it was not flagged before it
+ * is new, joins the flagged list and leaves the donor count. */
+__global__ __launch_bounds__(BLOCK) void
+grow_second(Params p, int iter)
+{
+	const unsigned n = p.nlist[nl_grow(p, iter)];
+	const unsigned stamp = pass_stamp(iter);
+	for (unsigned base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
+		unsigned e = base + threadIdx.x;
+		int f = 0, x = 0, y = 0, w = 1, h = 1;
+		long long off = 0;
+		if (e < n) {
+			unsigned long long entry = p.list_b[e];
+			f = (int)(entry >> 32);
+			int i = (int)(entry & 0xffffffffu);
+			const Frame fr = p.frames[f];
+			w = fr.w;
+			h = fr.h;
+			off = fr.off;
+			x = i % w;
+			y = i / w;
+			atomicAnd(&p.flags[off + i], ~GROW);
+		}
+		for (int k = 0; k < 9; ++k) {
+			int qx = x + k % 3 - 1, qy = y + k / 3 - 1;
+			bool fresh = false, lost = false;
+			int q = qy * w + qx;
+			if (e < n && qx >= 0 && qx < w && qy >= 0 && qy < h &&
+			    sprime_at(p, off + q, stamp) > p.neighbor) {
+				uint32_t old = atomicOr(&p.flags[off + q], CRAY);
+				if (!(old & CRAY)) {
+					fresh = true;
+					lost = !(old & EXCL) && isfinite(p.clean[off + q]);
+					atomicAdd(&p.todo[iter * p.nf + f], 1);
+					if (lost)
+						atomicSub(&p.counts[f].donors, 1);
+				}
+			}
+			list_push(fresh, list_entry(f, q), p.list, &p.nlist[0]);
 		}
 	}
 }
