@@ -150,11 +150,11 @@ device_cus()
 /* Frame table of a batch: offsets and tile numbering. */
 std::vector<Frame>
 frame_table(const cusmic_image *ims, size_t nf, const cusmic_options &o, const double *fg,
-	const double *frn, int *ntiles_out)
+	const double *frn, int *ntiles_out, int *nchunks_out = nullptr)
 {
 	std::vector<Frame> frames(nf);
 	long long off = 0;
-	int tiles = 0;
+	int tiles = 0, chunks = 0;
 	for (size_t f = 0; f < nf; ++f) {
 		const cusmic_image &im = ims[f];
 		Frame &fr = frames[f];
@@ -172,12 +172,18 @@ frame_table(const cusmic_image *ims, size_t nf, const cusmic_options &o, const d
 		fr.tiles_x = (fr.w + TILE_W - 1) / TILE_W;
 		fr.tiles_y = (fr.h + TILE_H - 1) / TILE_H;
 		fr.tile0 = tiles;
-		if ((long long)tiles + (long long)fr.tiles_x * fr.tiles_y > INT_MAX)
+		fr.chunk0 = chunks;
+		long long px = (long long)fr.w * fr.h;
+		if ((long long)tiles + (long long)fr.tiles_x * fr.tiles_y > INT_MAX ||
+			(long long)chunks + (px + CHUNK - 1) / CHUNK > INT_MAX)
 			throw std::invalid_argument("batch is too large");
 		tiles += fr.tiles_x * fr.tiles_y;
-		off += (long long)fr.w * fr.h;
+		chunks += int((px + CHUNK - 1) / CHUNK);
+		off += px;
 	}
 	*ntiles_out = tiles;
+	if (nchunks_out)
+		*nchunks_out = chunks;
 	return frames;
 }
 
@@ -224,8 +230,8 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	const double *frn, int laplacian, double *clean, uint8_t *mask, int32_t *donors,
 	int32_t *passes, hipStream_t stream, double *profile = nullptr)
 {
-	int ntiles = 0;
-	std::vector<Frame> frames = frame_table(ims, nf, o, fg, frn, &ntiles);
+	int ntiles = 0, nchunks = 0;
+	std::vector<Frame> frames = frame_table(ims, nf, o, fg, frn, &ntiles, &nchunks);
 	size_t total = 0;
 	for (const auto &fr : frames)
 		total += size_t(fr.w) * fr.h;
@@ -235,7 +241,7 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	size_t n_counts = nf * (sizeof(Counts) / 4);
 	size_t n_todo = size_t(std::max(maxiter, 1)) * nf;
 	size_t n_queue = size_t(std::max(maxiter, 1));
-	size_t n_nlist = 2 + 4 * size_t(std::max(maxiter, 1));
+	size_t n_nlist = 3 + 5 * size_t(std::max(maxiter, 1));
 	size_t n_dirty = 2 * size_t(ntiles);
 	size_t words = n_counts + n_todo + n_queue + n_nlist + 1 + n_dirty;
 	size_t off_frames = 0;
@@ -243,7 +249,8 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	size_t off_dlist = align_up(off_words + words * 4, 256);
 	size_t off_work = align_up(off_dlist + n_dirty * 4, 256);
 	size_t off_tframe = align_up(off_work + n_queue * 8, 256);
-	size_t off_flags = align_up(off_tframe + size_t(ntiles) * 4, 256);
+	size_t off_cframe = align_up(off_tframe + size_t(ntiles) * 4, 256);
+	size_t off_flags = align_up(off_cframe + size_t(nchunks) * 4, 256);
 	size_t planes = maxiter ? total : 0;
 	size_t off_sprime = align_up(off_flags + planes * 4, 256);
 	size_t off_list = align_up(off_sprime + planes * 8, 256);
@@ -279,14 +286,18 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	p.dirty_list = reinterpret_cast<unsigned int *>(base + off_dlist);
 	p.work = reinterpret_cast<unsigned long long *>(base + off_work);
 	p.tile_frame = reinterpret_cast<int *>(base + off_tframe);
+	p.chunk_frame = reinterpret_cast<int *>(base + off_cframe);
+	p.nchunks = nchunks;
 	p.flags = reinterpret_cast<uint32_t *>(base + off_flags);
 	p.sprime = reinterpret_cast<double *>(base + off_sprime);
 	p.list = reinterpret_cast<unsigned long long *>(base + off_list);
 	p.list_b = reinterpret_cast<unsigned long long *>(base + off_list_b);
 	p.list_c = reinterpret_cast<unsigned long long *>(base + off_list_c);
 
+	unsigned table_grid = unsigned(std::min<long long>(((long long)ntiles + nchunks + 255) / 256, 4096));
+	hipLaunchKernelGGL(set_tables, dim3(table_grid), dim3(256), 0, stream, p);
 	if (!maxiter) {
-		hipLaunchKernelGGL(copy_without_detection, dim3(ntiles), dim3(BLOCK), 0, stream, p);
+		hipLaunchKernelGGL(copy_without_detection, dim3(nchunks), dim3(BLOCK), 0, stream, p);
 		hip_check(hipGetLastError());
 		hip_check(hipStreamSynchronize(stream));
 		return;
@@ -302,10 +313,11 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	hipLaunchKernelGGL(zero_words, dim3(1), dim3(256), 0, stream,
 		reinterpret_cast<unsigned int *>(p.work), n_queue * 2);
 	tm.mark(-1);
-	hipLaunchKernelGGL(prologue, dim3(ntiles), dim3(BLOCK), 0, stream, p);
+	hipLaunchKernelGGL(prologue, dim3(nchunks), dim3(BLOCK), 0, stream, p);
 	tm.mark(CUSMIC_PROFILE_PROLOGUE_MS);
 	unsigned replace_grid = unsigned(cus) * 8;
 	hipLaunchKernelGGL(replace_pass, dim3(replace_grid), dim3(BLOCK), 0, stream, p, 0, 1);
+	hipLaunchKernelGGL(replace_widened, dim3(cus), dim3(BLOCK), 0, stream, p, 0, 1);
 	tm.mark(CUSMIC_PROFILE_REPLACE_MS);
 	unsigned sig_grid = unsigned(std::min(ntiles, cus * SIG_WGS_PER_CU));
 	unsigned list_grid = unsigned(cus) * 4;
@@ -320,9 +332,10 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 		hipLaunchKernelGGL(grow_second, dim3(list_grid), dim3(BLOCK), 0, stream, p, it);
 		tm.mark(CUSMIC_PROFILE_DETECT_MS);
 		hipLaunchKernelGGL(replace_pass, dim3(replace_grid), dim3(BLOCK), 0, stream, p, it, 0);
+		hipLaunchKernelGGL(replace_widened, dim3(cus), dim3(BLOCK), 0, stream, p, it, 0);
 		tm.mark(CUSMIC_PROFILE_REPLACE_MS);
 	}
-	hipLaunchKernelGGL(epilogue, dim3(ntiles), dim3(BLOCK), 0, stream, p);
+	hipLaunchKernelGGL(epilogue, dim3(nchunks), dim3(BLOCK), 0, stream, p);
 	tm.mark(CUSMIC_PROFILE_EPILOGUE_MS);
 	hip_check(hipGetLastError());
 

@@ -44,7 +44,11 @@ struct Frame {
 	long long off;         /* first pixel in the batch planes */
 	int w, h;
 	int tile0, tiles_x, tiles_y;
+	int chunk0;            /* first CHUNK-pixel chunk of the frame */
 };
+
+/* Pointwise passes stream frames in chunks of CHUNK consecutive pixels. */
+constexpr int CHUNK = 2048;
 
 /* Device counters of one call; zeroed by reset_counters. */
 struct Counts {
@@ -68,12 +72,15 @@ struct Params {
 	unsigned int *queue;             /* [maxiter] tile queue heads */
 	unsigned int *nlist;             /* [0] list length, [1] hole count,
 					  * [2 + iter] dirty tiles, [2 + M + iter] seeds,
-					  * [2 + 2M + iter] candidates, [2 + 3M + iter] grown */
+					  * [2 + 2M + iter] candidates, [2 + 3M + iter] grown,
+					  * [2 + 4M + iter] widened replacements (iter = M: fill) */
 	unsigned int *dirty;             /* [2][ntiles] dirty flags */
 	unsigned int *dirty_list;        /* [2][ntiles] dirty tile lists */
 	int *status;                     /* [0] error bits */
 	unsigned long long *work;        /* [maxiter] in-frame pixels of processed tiles */
-	int *tile_frame;                 /* [ntiles] frame of each tile (set by the prologue) */
+	int *tile_frame;                 /* [ntiles] frame of each tile */
+	int *chunk_frame;                /* [nchunks] frame of each chunk */
+	int nchunks;
 };
 
 constexpr int ERR_NO_DONORS = 1;
@@ -83,6 +90,7 @@ __device__ __forceinline__ int nl_dirty(const Params &p, int it) { return 2 + it
 __device__ __forceinline__ int nl_seed(const Params &p, int it) { return 2 + p.maxiter + it; }
 __device__ __forceinline__ int nl_cand(const Params &p, int it) { return 2 + 2 * p.maxiter + it; }
 __device__ __forceinline__ int nl_grow(const Params &p, int it) { return 2 + 3 * p.maxiter + it; }
+__device__ __forceinline__ int nl_wide(const Params &p, int it) { return 2 + 4 * p.maxiter + it; }
 
 /* scipy.ndimage border mapping of coordinate x on an axis of n samples;
  * -1 means a CONSTANT-mode zero (src/filters.cuh:24-44). */

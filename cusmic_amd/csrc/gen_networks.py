@@ -272,10 +272,12 @@ def main():
             return [np.median(x[o:o + k].reshape(k * k, -1), axis=0) for o in range(t)]
         check(net, outs, inputs, expect)
         parts.append(emit(f"strip_median{k}x{t}", net, outs, inputs, "r", "out"))
-    # replacement selector: lowest 13 of 24 sorted
-    net, outs, inputs = lower_half(24, 13)
-    check(net, outs, inputs, lambda x: list(np.sort(x, axis=0)[:13]))
-    parts.append(emit("sort_low13_of24", net, outs, inputs, "v", "out"))
+    # replacement selectors: lowest 13 of the 24 neighbours of a 5x5 window,
+    # lowest 25 of the 48 of a 7x7 window
+    for n, keep in ((24, 13), (48, 25)):
+        net, outs, inputs = lower_half(n, keep)
+        check(net, outs, inputs, lambda x, keep=keep: list(np.sort(x, axis=0)[:keep]))
+        parts.append(emit(f"sort_low{keep}_of{n}", net, outs, inputs, "v", "out"))
     Path(__file__).with_name("networks.inc").write_text("\n".join(parts))
 
 

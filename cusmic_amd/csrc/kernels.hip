@@ -52,6 +52,38 @@ lds_append(bool pred, T item, T *list, int *count)
 		list[base + __popcll(m & ((1ull << lane) - 1))] = item;
 }
 
+__device__ __forceinline__ void
+list_push(bool pred, unsigned long long item, unsigned long long *list, unsigned int *count)
+{
+	unsigned long long m = __ballot(pred);
+	if (!m)
+		return;
+	int lane = threadIdx.x & 63, leader = __ffsll((long long)m) - 1;
+	unsigned base = 0;
+	if (lane == leader)
+		base = atomicAdd(count, (unsigned)__popcll(m));
+	base = __shfl(base, leader, 64);
+	if (pred)
+		list[base + __popcll(m & ((1ull << lane) - 1))] = item;
+}
+
+/* Add the number of lanes with pred to per-frame counters, one atomic per
+ * distinct frame in the wave (counter of frame f at base + f * stride). */
+__device__ __forceinline__ void
+wave_frame_add(bool pred, int f, int *base, int stride, int sign)
+{
+	unsigned long long m = __ballot(pred);
+	int lane = threadIdx.x & 63;
+	while (m) {
+		int leader = __ffsll((long long)m) - 1;
+		int lf = __shfl(f, leader, 64);
+		unsigned long long same = __ballot(pred && f == lf);
+		if (lane == leader)
+			atomicAdd(base + (size_t)lf * stride, sign * __popcll(same));
+		m &= ~same;
+	}
+}
+
 __device__ __forceinline__ double
 positive(double v)
 {
@@ -121,34 +153,64 @@ tile_of(const Frame &fr, int tile)
 	return t;
 }
 
+/* tile -> frame and chunk -> frame tables (binary search once per entry) */
+__global__ void
+set_tables(Params p)
+{
+	for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < p.ntiles + p.nchunks;
+	     t += gridDim.x * blockDim.x) {
+		if (t < p.ntiles) {
+			p.tile_frame[t] = find_frame(p.frames, p.nf, t);
+		} else {
+			int c = t - p.ntiles, lo = 0, hi = p.nf - 1;
+			while (lo < hi) {
+				int mid = (lo + hi + 1) >> 1;
+				if (p.frames[mid].chunk0 <= c)
+					lo = mid;
+				else
+					hi = mid - 1;
+			}
+			p.chunk_frame[c] = lo;
+		}
+	}
+}
+
+/* Pointwise passes: one workgroup per chunk of CHUNK consecutive pixels of a
+ * frame, PER pixels per thread, every load issued before the first use. */
+constexpr int PER = CHUNK / BLOCK;
+
 /* Prologue (src/image.cuh:24-48): holes, exclusions, C = (hole ? 0 : D) + bg,
  * per-frame counts of holes, allowed pixels and finite donors, and the list of
- * holes for the fill. One workgroup per tile. */
+ * holes for the fill. */
 __global__ __launch_bounds__(BLOCK) void
 prologue(Params p)
 {
 	__shared__ int red[4];
-	int tile = blockIdx.x;
-	int f = find_frame(p.frames, p.nf, tile);
+	const int f = p.chunk_frame[blockIdx.x];
 	const Frame fr = p.frames[f];
-	TilePixels t = tile_of(fr, tile);
-	bool bg_map = fr.background != nullptr, bg = bg_map || !isnan(p.bg);
+	const int n = fr.w * fr.h, base = (blockIdx.x - fr.chunk0) * CHUNK + threadIdx.x;
+	const bool bg_map = fr.background != nullptr, bg = bg_map || !isnan(p.bg);
+	double d[PER], b[PER];
+	uint8_t m[PER];
+#pragma unroll
+	for (int k = 0; k < PER; ++k) {
+		int i = base + k * BLOCK;
+		bool in = i < n;
+		d[k] = in ? fr.data[i] : 0.0;
+		b[k] = in && bg_map ? fr.background[i] : p.bg;
+		m[k] = in && fr.mask ? fr.mask[i] : 0;
+	}
 	int holes = 0, allowed = 0, donors = 0;
-	if (!threadIdx.x)
-		p.tile_frame[tile] = f;
-	int x = t.x0 + (threadIdx.x & 63);
-	for (int r = threadIdx.x >> 6; r < TILE_H; r += 4) {
-		int y = t.y0 + r;
-		bool in = x < t.w && y < t.h;
-		bool hole = false;
-		int i = y * t.w + x;
+#pragma unroll
+	for (int k = 0; k < PER; ++k) {
+		int i = base + k * BLOCK;
+		bool in = i < n;
+		bool hole = in && !isfinite(d[k]);
 		if (in) {
-			double d = fr.data[i];
-			hole = !isfinite(d);
-			bool allow = !hole && !(fr.mask && fr.mask[i]);
-			double c = hole ? 0.0 : d;
+			bool allow = !hole && !m[k];
+			double c = hole ? 0.0 : d[k];
 			if (bg)
-				c += 1 * (bg_map ? fr.background[i] : p.bg);
+				c += 1 * b[k];
 			long long g = fr.off + i;
 			p.clean[g] = c;
 			p.flags[g] = (hole ? HOLE : 0u) | (allow ? 0u : EXCL);
@@ -161,15 +223,15 @@ prologue(Params p)
 			donors += allow && isfinite(c);
 		}
 		/* hole list (only consulted when maxiter > 0) */
-		unsigned long long m = __ballot(hole);
-		if (m) {
-			int lane = threadIdx.x & 63, leader = __ffsll((long long)m) - 1;
-			unsigned base = 0;
+		unsigned long long mk = __ballot(hole);
+		if (mk) {
+			int lane = threadIdx.x & 63, leader = __ffsll((long long)mk) - 1;
+			unsigned slot = 0;
 			if (lane == leader)
-				base = atomicAdd(&p.nlist[1], (unsigned)__popcll(m));
-			base = __shfl(base, leader, 64);
+				slot = atomicAdd(&p.nlist[1], (unsigned)__popcll(mk));
+			slot = __shfl(slot, leader, 64);
 			if (hole)
-				p.list[base + __popcll(m & ((1ull << lane) - 1))] = list_entry(f, i);
+				p.list[slot + __popcll(mk & ((1ull << lane) - 1))] = list_entry(f, i);
 		}
 	}
 	holes = block_sum(holes, red);
@@ -189,17 +251,15 @@ prologue(Params p)
 __global__ __launch_bounds__(BLOCK) void
 copy_without_detection(Params p)
 {
-	int tile = blockIdx.x;
-	int f = find_frame(p.frames, p.nf, tile);
+	const int f = p.chunk_frame[blockIdx.x];
 	const Frame fr = p.frames[f];
-	TilePixels t = tile_of(fr, tile);
-	bool bg_map = fr.background != nullptr, bg = bg_map || !isnan(p.bg);
-	int x = t.x0 + (threadIdx.x & 63);
-	for (int r = threadIdx.x >> 6; r < TILE_H; r += 4) {
-		int y = t.y0 + r;
-		if (x >= t.w || y >= t.h)
+	const int n = fr.w * fr.h, base = (blockIdx.x - fr.chunk0) * CHUNK + threadIdx.x;
+	const bool bg_map = fr.background != nullptr, bg = bg_map || !isnan(p.bg);
+#pragma unroll
+	for (int k = 0; k < PER; ++k) {
+		int i = base + k * BLOCK;
+		if (i >= n)
 			continue;
-		int i = y * t.w + x;
 		double d = fr.data[i];
 		double v = isfinite(d) ? d : 0.0;
 		if (bg) {
@@ -217,24 +277,27 @@ copy_without_detection(Params p)
 __global__ __launch_bounds__(BLOCK) void
 epilogue(Params p)
 {
-	int tile = blockIdx.x;
-	int f = find_frame(p.frames, p.nf, tile);
+	const int f = p.chunk_frame[blockIdx.x];
 	const Frame fr = p.frames[f];
-	TilePixels t = tile_of(fr, tile);
-	bool bg_map = fr.background != nullptr, bg = bg_map || !isnan(p.bg);
-	int x = t.x0 + (threadIdx.x & 63);
-	for (int r = threadIdx.x >> 6; r < TILE_H; r += 4) {
-		int y = t.y0 + r;
-		if (x >= t.w || y >= t.h)
+	const int n = fr.w * fr.h, base = (blockIdx.x - fr.chunk0) * CHUNK + threadIdx.x;
+	const bool bg_map = fr.background != nullptr, bg = bg_map || !isnan(p.bg);
+	uint32_t fl[PER];
+#pragma unroll
+	for (int k = 0; k < PER; ++k) {
+		int i = base + k * BLOCK;
+		fl[k] = i < n ? p.flags[fr.off + i] : 0u;
+	}
+#pragma unroll
+	for (int k = 0; k < PER; ++k) {
+		int i = base + k * BLOCK;
+		if (i >= n)
 			continue;
-		int i = y * t.w + x;
 		long long g = fr.off + i;
-		uint32_t fl = p.flags[g];
-		if (fl & HOLE)
+		if (fl[k] & HOLE)
 			p.clean[g] = fr.data[i];
 		else if (bg)
 			p.clean[g] = p.clean[g] + -1 * (bg_map ? fr.background[i] : p.bg);
-		p.mask[g] = (fl & CRAY) ? 1 : 0;
+		p.mask[g] = (fl[k] & CRAY) ? 1 : 0;
 	}
 }
 
@@ -363,89 +426,33 @@ mark_dirty(const Params &p, const Frame &fr, int x, int y, int next)
 		}
 }
 
-/* Replace pixel i of frame fr from the donors of its smallest non-empty
- * clipped window. Returns true if the stored bits changed. */
+/* Value of a replacement from the two middle donors (src/replace.cuh:141-150):
+ * odd count -> 0.0 + lo; even -> overflow-safe midpoint. When both middles are
+ * zeros, the key order (-0 below +0) decides the sign: -0 iff more than n/2
+ * donors have a key at most key(-0) (neg0 counts them). */
+__device__ __forceinline__ double
+replacement(int n, double lo, double hi, int neg0)
+{
+	if (n & 1)
+		return 0.0 + lo;
+	if (lo == 0.0 && hi == 0.0)
+		return neg0 > (n >> 1) ? -0.0 : 0.0;
+	return midpoint(lo, hi);
+}
+
+/* Store the replacement of pixel i; record the donors when asked (ties in
+ * raster order within the final window). Returns true if the bits changed. */
 __device__ bool
-replace_pixel(const Params &p, const Frame &fr, int i, int32_t *donors)
+store_replacement(const Params &p, const Frame &fr, int i, int32_t *donors, int n, double lo,
+	double hi, int neg0, int x0, int x1, int y0, int y1)
 {
 	const double *clean = p.clean + fr.off;
 	const uint32_t *flags = p.flags + fr.off;
-	int w = fr.w, h = fr.h, x = i % w, y = i / w;
-	/* flags and values of the 24 neighbours are loaded together (one memory
-	 * round trip); a value read at a flagged neighbour, which another lane may
-	 * be rewriting, is discarded */
-	double v[24];
-	uint32_t fl[24];
-	int n = 0, neg0 = 0;
-#pragma unroll
-	for (int k = 0; k < 24; ++k) {
-		int o = k < 12 ? k : k + 1, qx = x + o % 5 - 2, qy = y + o / 5 - 2;
-		bool in = qx >= 0 && qx < w && qy >= 0 && qy < h;
-		int q = in ? qy * w + qx : i;
-		fl[k] = in ? flags[q] : EXCL;
-		v[k] = clean[q];
-	}
-#pragma unroll
-	for (int k = 0; k < 24; ++k) {
-		bool ok = !(fl[k] & (EXCL | CRAY)) && isfinite(v[k]);
-		v[k] = ok ? v[k] : INFINITY;
-		n += ok;
-		neg0 += ok && (v[k] < 0 || (v[k] == 0.0 && signbit(v[k])));
-	}
-	double lo, hi;
-	int r = 2;
-	int x0 = max(0, x - 2), x1 = min(w - 1, x + 2), y0 = max(0, y - 2), y1 = min(h - 1, y + 2);
-	if (n) {
-		double s[13];
-		sort_low13_of24(v, s);
-		int a = (n - 1) >> 1, b = n >> 1;
-		lo = s[0];
-		hi = s[0];
-#pragma unroll
-		for (int j = 1; j < 13; ++j) {
-			lo = j == a ? s[j] : lo;
-			hi = j == b ? s[j] : hi;
-		}
-	} else {
-		/* rare: widen the window until it holds a donor */
-		int rmax = max(w, h);
-		for (r = 3; r <= rmax; ++r) {
-			x0 = max(0, x - r);
-			x1 = min(w - 1, x + r);
-			y0 = max(0, y - r);
-			y1 = min(h - 1, y + r);
-			n = 0;
-			for (int yy = y0; yy <= y1; ++yy)
-				for (int xx = x0; xx <= x1; ++xx) {
-					double t;
-					n += is_donor(flags, clean, yy * w + xx, t);
-				}
-			if (n)
-				break;
-		}
-		if (!n)
-			return false; /* unreachable: callers check the frame has donors */
-		lo = key_value(window_select(flags, clean, w, x0, x1, y0, y1, (n - 1) >> 1));
-		hi = n & 1 ? lo : key_value(window_select(flags, clean, w, x0, x1, y0, y1, n >> 1));
-		neg0 = 0;
-		for (int yy = y0; yy <= y1; ++yy)
-			for (int xx = x0; xx <= x1; ++xx) {
-				double t;
-				if (is_donor(flags, clean, yy * w + xx, t))
-					neg0 += t < 0 || (t == 0.0 && signbit(t));
-			}
-	}
-	double out;
-	if (n & 1)
-		out = 0.0 + lo;
-	else if (lo == 0.0 && hi == 0.0)
-		out = neg0 > (n >> 1) ? -0.0 : 0.0;
-	else
-		out = midpoint(lo, hi);
+	double out = replacement(n, lo, hi, neg0);
 	if (donors) {
 		int a = (n - 1) >> 1, b = n >> 1;
-		int ia = rank_index(flags, clean, w, x0, x1, y0, y1, rank_key(lo, a, neg0), a);
-		int ib = a == b ? ia : rank_index(flags, clean, w, x0, x1, y0, y1, rank_key(hi, b, neg0), b);
+		int ia = rank_index(flags, clean, fr.w, x0, x1, y0, y1, rank_key(lo, a, neg0), a);
+		int ib = a == b ? ia : rank_index(flags, clean, fr.w, x0, x1, y0, y1, rank_key(hi, b, neg0), b);
 		donors[2 * (fr.off + i)] = ia;
 		donors[2 * (fr.off + i) + 1] = ib;
 	}
@@ -455,36 +462,213 @@ replace_pixel(const Params &p, const Frame &fr, int i, int32_t *donors)
 	return __double_as_longlong(old) != __double_as_longlong(out);
 }
 
+/* Gather the K neighbours of the (2R+1)^2 window around (x, y) (centre
+ * excluded): donor values, +inf elsewhere. Flags and values are loaded together
+ * (one memory round trip); a value read at a flagged neighbour, which another
+ * lane may be rewriting, is discarded. */
+template <int R, int K>
+__device__ __forceinline__ int
+gather_window(const Params &p, const Frame &fr, int x, int y, double (&v)[K], int &neg0)
+{
+	constexpr int S = 2 * R + 1, C = R * S + R;
+	const double *clean = p.clean + fr.off;
+	const uint32_t *flags = p.flags + fr.off;
+	const int w = fr.w, h = fr.h, i = y * w + x;
+	uint32_t fl[K];
+#pragma unroll
+	for (int k = 0; k < K; ++k) {
+		int o = k < C ? k : k + 1, qx = x + o % S - R, qy = y + o / S - R;
+		bool in = qx >= 0 && qx < w && qy >= 0 && qy < h;
+		int q = in ? qy * w + qx : i;
+		fl[k] = in ? flags[q] : EXCL;
+		v[k] = clean[q];
+	}
+	int n = 0;
+	neg0 = 0;
+#pragma unroll
+	for (int k = 0; k < K; ++k) {
+		bool ok = !(fl[k] & (EXCL | CRAY)) && isfinite(v[k]);
+		v[k] = ok ? v[k] : INFINITY;
+		n += ok;
+		neg0 += ok && (v[k] < 0 || (v[k] == 0.0 && signbit(v[k])));
+	}
+	return n;
+}
+
+/* Replacement from the 5x5 window (src/replace.cuh:118-150 with r = 2).
+ * Returns 1 if the pixel changed, 0 if not, -1 if the window has no donor. */
+__device__ int
+replace_near(const Params &p, const Frame &fr, int i, int32_t *donors)
+{
+	const int w = fr.w, h = fr.h, x = i % w, y = i / w;
+	double v[24], s[13];
+	int neg0;
+	int n = gather_window<2, 24>(p, fr, x, y, v, neg0);
+	if (!n)
+		return -1;
+	sort_low13_of24(v, s);
+	int a = (n - 1) >> 1, b = n >> 1;
+	double lo = s[0], hi = s[0];
+#pragma unroll
+	for (int j = 1; j < 13; ++j) {
+		lo = j == a ? s[j] : lo;
+		hi = j == b ? s[j] : hi;
+	}
+	return store_replacement(p, fr, i, donors, n, lo, hi, neg0, max(0, x - 2), min(w - 1, x + 2),
+		max(0, y - 2), min(h - 1, y + 2));
+}
+
+/* rank-th smallest of n keys in a private array (MSB-first bit search) */
+__device__ unsigned long long
+array_select(const unsigned long long *keys, int n, int rank)
+{
+	unsigned long long prefix = 0;
+	for (int bit = 63; bit >= 0; --bit) {
+		unsigned long long hi_mask = bit == 63 ? 0 : ~0ull << (bit + 1);
+		int below = 0;
+		for (int j = 0; j < n; ++j)
+			below += (keys[j] & hi_mask) == (prefix & hi_mask) && !((keys[j] >> bit) & 1);
+		if (rank >= below) {
+			prefix |= 1ull << bit;
+			rank -= below;
+		}
+	}
+	return prefix;
+}
+
+constexpr int WIDE_KEYS = 288; /* donors of windows up to 17 x 17 */
+
+/* Replacement from a widened window, r = 3, 4, ... (a 5x5 window without
+ * donors). Returns true if the pixel changed. */
+__device__ bool
+replace_wide(const Params &p, const Frame &fr, int i, int32_t *donors)
+{
+	const double *clean = p.clean + fr.off;
+	const uint32_t *flags = p.flags + fr.off;
+	const int w = fr.w, h = fr.h, x = i % w, y = i / w;
+	{
+		double v[48], s[25];
+		int neg0;
+		int n = gather_window<3, 48>(p, fr, x, y, v, neg0);
+		if (n) {
+			sort_low25_of48(v, s);
+			int a = (n - 1) >> 1, b = n >> 1;
+			double lo = s[0], hi = s[0];
+#pragma unroll
+			for (int j = 1; j < 25; ++j) {
+				lo = j == a ? s[j] : lo;
+				hi = j == b ? s[j] : hi;
+			}
+			return store_replacement(p, fr, i, donors, n, lo, hi, neg0, max(0, x - 3),
+				min(w - 1, x + 3), max(0, y - 3), min(h - 1, y + 3));
+		}
+	}
+	int rmax = max(w, h), n = 0, x0 = 0, x1 = 0, y0 = 0, y1 = 0;
+	for (int r = 4; r <= rmax; ++r) {
+		x0 = max(0, x - r);
+		x1 = min(w - 1, x + r);
+		y0 = max(0, y - r);
+		y1 = min(h - 1, y + r);
+		n = 0;
+		for (int yy = y0; yy <= y1; ++yy)
+			for (int xx = x0; xx <= x1; ++xx) {
+				double t;
+				n += is_donor(flags, clean, yy * w + xx, t);
+			}
+		if (n)
+			break;
+	}
+	if (!n)
+		return false; /* unreachable: callers check the frame has donors */
+	double lo, hi;
+	int neg0 = 0;
+	if (n <= WIDE_KEYS) {
+		unsigned long long keys[WIDE_KEYS];
+		int k = 0;
+		for (int yy = y0; yy <= y1; ++yy)
+			for (int xx = x0; xx <= x1; ++xx) {
+				double t;
+				if (is_donor(flags, clean, yy * w + xx, t)) {
+					keys[k++] = order_key(t);
+					neg0 += t < 0 || (t == 0.0 && signbit(t));
+				}
+			}
+		lo = key_value(array_select(keys, n, (n - 1) >> 1));
+		hi = n & 1 ? lo : key_value(array_select(keys, n, n >> 1));
+	} else {
+		lo = key_value(window_select(flags, clean, w, x0, x1, y0, y1, (n - 1) >> 1));
+		hi = n & 1 ? lo : key_value(window_select(flags, clean, w, x0, x1, y0, y1, n >> 1));
+		for (int yy = y0; yy <= y1; ++yy)
+			for (int xx = x0; xx <= x1; ++xx) {
+				double t;
+				if (is_donor(flags, clean, yy * w + xx, t))
+					neg0 += t < 0 || (t == 0.0 && signbit(t));
+			}
+	}
+	return store_replacement(p, fr, i, donors, n, lo, hi, neg0, x0, x1, y0, y1);
+}
+
+/* May the pixels of frame f be replaced in this pass? Sets the no-donor error. */
+__device__ __forceinline__ bool
+replace_allowed(const Params &p, int f, int iter, int fill)
+{
+	Counts c = p.counts[f];
+	if (fill) {
+		if (!c.allowed)
+			return false;
+		if (!c.donors) {
+			atomicOr(p.status, ERR_NO_DONORS);
+			return false;
+		}
+		return true;
+	}
+	if (!p.todo[iter * p.nf + f])
+		return false;
+	if (c.donors <= 0) {
+		atomicOr(p.status, ERR_NO_DONORS);
+		return false;
+	}
+	return true;
+}
+
 /* Hole fill before the loop (fill = 1, src/cleaner.cuh:78-100) or the
- * replacement pass of iteration iter over every flagged pixel. */
+ * replacement pass of iteration iter over every flagged pixel. Pixels whose
+ * 5x5 window has no donor go to the widened list. */
 __global__ __launch_bounds__(BLOCK) void
 replace_pass(Params p, int iter, int fill)
 {
-	unsigned n = fill ? p.nlist[1] : p.nlist[0];
+	const unsigned n = fill ? p.nlist[1] : p.nlist[0];
+	unsigned int *wide = &p.nlist[nl_wide(p, fill ? p.maxiter : iter)];
+	for (unsigned base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
+		unsigned e = base + threadIdx.x;
+		bool widen = false;
+		unsigned long long entry = 0;
+		if (e < n) {
+			entry = p.list[e];
+			int f = (int)(entry >> 32), i = (int)(entry & 0xffffffffu);
+			if (replace_allowed(p, f, iter, fill)) {
+				const Frame fr = p.frames[f];
+				int r = replace_near(p, fr, i, fill ? nullptr : p.donors);
+				widen = r < 0;
+				if (r > 0 && !fill && iter + 1 < p.maxiter)
+					mark_dirty(p, fr, i % fr.w, i / fr.w, iter + 1);
+			}
+		}
+		list_push(widen, entry, p.list_c, wide);
+	}
+}
+
+__global__ __launch_bounds__(BLOCK) void
+replace_widened(Params p, int iter, int fill)
+{
+	const unsigned n = p.nlist[nl_wide(p, fill ? p.maxiter : iter)];
 	for (unsigned e = blockIdx.x * BLOCK + threadIdx.x; e < n; e += gridDim.x * BLOCK) {
-		unsigned long long entry = p.list[e];
+		unsigned long long entry = p.list_c[e];
 		int f = (int)(entry >> 32), i = (int)(entry & 0xffffffffu);
 		const Frame fr = p.frames[f];
-		Counts c = p.counts[f];
-		if (fill) {
-			if (!c.allowed)
-				continue;
-			if (!c.donors) {
-				atomicOr(p.status, ERR_NO_DONORS);
-				continue;
-			}
-			replace_pixel(p, fr, i, nullptr);
-		} else {
-			if (!p.todo[iter * p.nf + f])
-				continue;
-			if (c.donors <= 0) {
-				atomicOr(p.status, ERR_NO_DONORS);
-				continue;
-			}
-			bool changed = replace_pixel(p, fr, i, p.donors);
-			if (changed && iter + 1 < p.maxiter)
-				mark_dirty(p, fr, i % fr.w, i / fr.w, iter + 1);
-		}
+		bool changed = replace_wide(p, fr, i, fill ? nullptr : p.donors);
+		if (changed && !fill && iter + 1 < p.maxiter)
+			mark_dirty(p, fr, i % fr.w, i / fr.w, iter + 1);
 	}
 }
 
@@ -562,20 +746,6 @@ map_index(int x, int y, int w, int h, int mode)
 	return y * w + x;
 }
 
-__device__ __forceinline__ void
-list_push(bool pred, unsigned long long item, unsigned long long *list, unsigned int *count)
-{
-	unsigned long long m = __ballot(pred);
-	if (!m)
-		return;
-	int lane = threadIdx.x & 63, leader = __ffsll((long long)m) - 1;
-	unsigned base = 0;
-	if (lane == leader)
-		base = atomicAdd(count, (unsigned)__popcll(m));
-	base = __shfl(base, leader, 64);
-	if (pred)
-		list[base + __popcll(m & ((1ull << lane) - 1))] = item;
-}
 
 struct SigShared {
 	double s[SH][SW];
@@ -914,12 +1084,11 @@ grow_second(Params p, int iter)
 				if (!(old & CRAY)) {
 					fresh = true;
 					lost = !(old & EXCL) && isfinite(p.clean[off + q]);
-					atomicAdd(&p.todo[iter * p.nf + f], 1);
-					if (lost)
-						atomicSub(&p.counts[f].donors, 1);
 				}
 			}
 			list_push(fresh, list_entry(f, q), p.list, &p.nlist[0]);
+			wave_frame_add(fresh, f, p.todo + iter * p.nf, 1, 1);
+			wave_frame_add(lost, f, &p.counts[0].donors, int(sizeof(Counts) / sizeof(int)), -1);
 		}
 	}
 }
@@ -936,9 +1105,10 @@ donor_reach(const Frame *frames, int nf, const uint8_t *mask, const int32_t *don
 	const Frame fr = frames[f];
 	TilePixels t = tile_of(fr, tile);
 	int x = t.x0 + (threadIdx.x & 63), r = 0;
+	const bool col = (threadIdx.x & 63) < TILE_W;
 	for (int row = threadIdx.x >> 6; row < TILE_H; row += 4) {
 		int y = t.y0 + row;
-		if (x >= t.w || y >= t.h)
+		if (!col || x >= t.w || y >= t.h)
 			continue;
 		long long g = fr.off + (long long)y * t.w + x;
 		if (!mask[g])
@@ -969,9 +1139,10 @@ adjoint_gather(const Frame *frames, int nf, const uint8_t *mask, const int32_t *
 	TilePixels t = tile_of(fr, tile);
 	const int R = max(2, reach[f]);
 	int x = t.x0 + (threadIdx.x & 63);
+	const bool col = (threadIdx.x & 63) < TILE_W; /* lanes past the tile width idle */
 	for (int row = threadIdx.x >> 6; row < TILE_H; row += 4) {
 		int y = t.y0 + row;
-		if (x >= t.w || y >= t.h)
+		if (!col || x >= t.w || y >= t.h)
 			continue;
 		int q = y * t.w + x;
 		const long long base = fr.off;
