@@ -28,8 +28,9 @@
 
 using namespace cusmic;
 
-/* Persistent significance workgroups per CU (LDS 25 KB, 4 waves each). */
-constexpr int SIG_WGS_PER_CU = 4;
+/* Persistent significance workgroups per CU (LDS 29 KB, 4 waves each, at
+ * most 5 waves per SIMD with 97 VGPRs). */
+constexpr int SIG_WGS_PER_CU = 5;
 
 namespace {
 

@@ -48,7 +48,7 @@ struct Frame {
 };
 
 /* Pointwise passes stream frames in chunks of CHUNK consecutive pixels. */
-constexpr int CHUNK = 2048;
+constexpr int CHUNK = 8192;
 
 /* Device counters of one call; zeroed by reset_counters. */
 struct Counts {

@@ -176,8 +176,9 @@ set_tables(Params p)
 }
 
 /* Pointwise passes: one workgroup per chunk of CHUNK consecutive pixels of a
- * frame, PER pixels per thread, every load issued before the first use. */
-constexpr int PER = CHUNK / BLOCK;
+ * frame, in batches of PER pixels per thread whose loads are all issued
+ * before the first use. */
+constexpr int PER = 8;
 
 /* Prologue (src/image.cuh:24-48): holes, exclusions, C = (hole ? 0 : D) + bg,
  * per-frame counts of holes, allowed pixels and finite donors, and the list of
@@ -188,50 +189,43 @@ prologue(Params p)
 	__shared__ int red[4];
 	const int f = p.chunk_frame[blockIdx.x];
 	const Frame fr = p.frames[f];
-	const int n = fr.w * fr.h, base = (blockIdx.x - fr.chunk0) * CHUNK + threadIdx.x;
+	const int n = fr.w * fr.h, start = (blockIdx.x - fr.chunk0) * CHUNK + threadIdx.x;
 	const bool bg_map = fr.background != nullptr, bg = bg_map || !isnan(p.bg);
-	double d[PER], b[PER];
-	uint8_t m[PER];
-#pragma unroll
-	for (int k = 0; k < PER; ++k) {
-		int i = base + k * BLOCK;
-		bool in = i < n;
-		d[k] = in ? fr.data[i] : 0.0;
-		b[k] = in && bg_map ? fr.background[i] : p.bg;
-		m[k] = in && fr.mask ? fr.mask[i] : 0;
-	}
 	int holes = 0, allowed = 0, donors = 0;
+	for (int base = start; base < start + CHUNK; base += PER * BLOCK) {
+		double d[PER], b[PER];
+		uint8_t m[PER];
 #pragma unroll
-	for (int k = 0; k < PER; ++k) {
-		int i = base + k * BLOCK;
-		bool in = i < n;
-		bool hole = in && !isfinite(d[k]);
-		if (in) {
-			bool allow = !hole && !m[k];
-			double c = hole ? 0.0 : d[k];
-			if (bg)
-				c += 1 * b[k];
-			long long g = fr.off + i;
-			p.clean[g] = c;
-			p.flags[g] = (hole ? HOLE : 0u) | (allow ? 0u : EXCL);
-			if (p.donors) {
-				p.donors[2 * g] = -1;
-				p.donors[2 * g + 1] = -1;
-			}
-			holes += hole;
-			allowed += allow;
-			donors += allow && isfinite(c);
+		for (int k = 0; k < PER; ++k) {
+			int i = base + k * BLOCK;
+			bool in = i < n;
+			d[k] = in ? fr.data[i] : 0.0;
+			b[k] = in && bg_map ? fr.background[i] : p.bg;
+			m[k] = in && fr.mask ? fr.mask[i] : 0;
 		}
-		/* hole list (only consulted when maxiter > 0) */
-		unsigned long long mk = __ballot(hole);
-		if (mk) {
-			int lane = threadIdx.x & 63, leader = __ffsll((long long)mk) - 1;
-			unsigned slot = 0;
-			if (lane == leader)
-				slot = atomicAdd(&p.nlist[1], (unsigned)__popcll(mk));
-			slot = __shfl(slot, leader, 64);
-			if (hole)
-				p.list[slot + __popcll(mk & ((1ull << lane) - 1))] = list_entry(f, i);
+#pragma unroll
+		for (int k = 0; k < PER; ++k) {
+			int i = base + k * BLOCK;
+			bool in = i < n;
+			bool hole = in && !isfinite(d[k]);
+			if (in) {
+				bool allow = !hole && !m[k];
+				double c = hole ? 0.0 : d[k];
+				if (bg)
+					c += 1 * b[k];
+				long long g = fr.off + i;
+				p.clean[g] = c;
+				p.flags[g] = (hole ? HOLE : 0u) | (allow ? 0u : EXCL);
+				if (p.donors) {
+					p.donors[2 * g] = -1;
+					p.donors[2 * g + 1] = -1;
+				}
+				holes += hole;
+				allowed += allow;
+				donors += allow && isfinite(c);
+			}
+			/* hole list (only consulted when maxiter > 0) */
+			list_push(hole, list_entry(f, i), p.list, &p.nlist[1]);
 		}
 	}
 	holes = block_sum(holes, red);
@@ -253,13 +247,9 @@ copy_without_detection(Params p)
 {
 	const int f = p.chunk_frame[blockIdx.x];
 	const Frame fr = p.frames[f];
-	const int n = fr.w * fr.h, base = (blockIdx.x - fr.chunk0) * CHUNK + threadIdx.x;
+	const int n = fr.w * fr.h, start = (blockIdx.x - fr.chunk0) * CHUNK + threadIdx.x;
 	const bool bg_map = fr.background != nullptr, bg = bg_map || !isnan(p.bg);
-#pragma unroll
-	for (int k = 0; k < PER; ++k) {
-		int i = base + k * BLOCK;
-		if (i >= n)
-			continue;
+	for (int i = start; i < start + CHUNK && i < n; i += BLOCK) {
 		double d = fr.data[i];
 		double v = isfinite(d) ? d : 0.0;
 		if (bg) {
@@ -279,25 +269,27 @@ epilogue(Params p)
 {
 	const int f = p.chunk_frame[blockIdx.x];
 	const Frame fr = p.frames[f];
-	const int n = fr.w * fr.h, base = (blockIdx.x - fr.chunk0) * CHUNK + threadIdx.x;
+	const int n = fr.w * fr.h, start = (blockIdx.x - fr.chunk0) * CHUNK + threadIdx.x;
 	const bool bg_map = fr.background != nullptr, bg = bg_map || !isnan(p.bg);
-	uint32_t fl[PER];
+	for (int base = start; base < start + CHUNK; base += PER * BLOCK) {
+		uint32_t fl[PER];
 #pragma unroll
-	for (int k = 0; k < PER; ++k) {
-		int i = base + k * BLOCK;
-		fl[k] = i < n ? p.flags[fr.off + i] : 0u;
-	}
+		for (int k = 0; k < PER; ++k) {
+			int i = base + k * BLOCK;
+			fl[k] = i < n ? p.flags[fr.off + i] : 0u;
+		}
 #pragma unroll
-	for (int k = 0; k < PER; ++k) {
-		int i = base + k * BLOCK;
-		if (i >= n)
-			continue;
-		long long g = fr.off + i;
-		if (fl[k] & HOLE)
-			p.clean[g] = fr.data[i];
-		else if (bg)
-			p.clean[g] = p.clean[g] + -1 * (bg_map ? fr.background[i] : p.bg);
-		p.mask[g] = (fl[k] & CRAY) ? 1 : 0;
+		for (int k = 0; k < PER; ++k) {
+			int i = base + k * BLOCK;
+			if (i >= n)
+				continue;
+			long long g = fr.off + i;
+			if (fl[k] & HOLE)
+				p.clean[g] = fr.data[i];
+			else if (bg)
+				p.clean[g] = p.clean[g] + -1 * (bg_map ? fr.background[i] : p.bg);
+			p.mask[g] = (fl[k] & CRAY) ? 1 : 0;
+		}
 	}
 }
 
@@ -747,11 +739,31 @@ map_index(int x, int y, int w, int h, int mode)
 }
 
 
+constexpr int SEED_BUF = 512; /* seeds buffered per workgroup between flushes */
+
 struct SigShared {
 	double s[SH][SW];
 	uint16_t alist[PW * PH];
-	int na, tile, unsafe;
+	unsigned long long seeds[SEED_BUF];
+	int na, tile, unsafe, nseed;
+	unsigned flush_base;
 };
+
+/* Move the buffered seeds of this workgroup to the global list. */
+__device__ void
+flush_seeds(SigShared &sh, unsigned long long *list, unsigned int *count)
+{
+	__syncthreads();
+	const int n = sh.nseed;
+	if (!threadIdx.x && n)
+		sh.flush_base = atomicAdd(count, (unsigned)n);
+	__syncthreads();
+	for (int k = threadIdx.x; k < n; k += BLOCK)
+		list[sh.flush_base + k] = sh.seeds[k];
+	__syncthreads();
+	if (!threadIdx.x)
+		sh.nseed = 0;
+}
 
 template <bool MODEL>
 __global__ __launch_bounds__(BLOCK) void
@@ -768,8 +780,10 @@ significance(Params p, int iter)
 	const unsigned int *dlist = p.dirty_list + (size_t)(iter & 1) * p.ntiles;
 	unsigned int *seeds = &p.nlist[nl_seed(p, iter)];
 	unsigned next = 0;
-	if (!tid)
+	if (!tid) {
 		next = atomicAdd(&p.queue[iter], 1u);
+		sh.nseed = 0;
+	}
 
 	for (;;) {
 		__syncthreads();
@@ -802,7 +816,24 @@ significance(Params p, int iter)
 		const int gx = X0 - HS + lane;
 		const int r0 = wave * ROWS_PER_WAVE;
 		bool bad = false;
-		if (!MODEL || fr.error) {
+		/* the region and its one-pixel ring lie inside the frame */
+		const bool inner = X0 - HS - 1 >= 0 && X0 - HS + SW < w && Y0 - HS - 1 >= 0 && Y0 - HS + SH < h;
+		if ((!MODEL || fr.error) && inner) {
+			/* interior tile: direct row pointers, no border logic */
+			const double *rp = clean + (long long)(Y0 - HS + r0) * w + gx;
+			const double *ep = fr.error + (long long)(Y0 - HS + r0) * w + gx;
+			double up = rp[-w], cur = rp[0];
+#pragma unroll 5
+			for (int r = 0; r < ROWS_PER_WAVE; ++r) {
+				double dn = rp[w], l = rp[-1], rt = rp[1], nz = *ep;
+				bad |= !(nz > 0) && !isnan(nz);
+				sh.s[r0 + r][lane] = laplacian(cur, up, dn, l, rt, exact, narrow) / (2 * nz);
+				up = cur;
+				cur = dn;
+				rp += w;
+				ep += w;
+			}
+		} else if (!MODEL || fr.error) {
 			double up = lap_sample(clean, gx, Y0 - HS + r0 - 1, w, h, mode);
 			double cur = lap_sample(clean, gx, Y0 - HS + r0, w, h, mode);
 #pragma unroll 5
@@ -882,10 +913,15 @@ significance(Params p, int iter)
 		}
 		__syncthreads();
 
-		/* 4. S' at the screened pixels (holes excluded: S' = 0 there), stamped
-		 * into the sparse plane; seeds to the list */
+		/* 4. S' at the screened pixels, stamped into the sparse plane; those
+		 * above t are buffered as seeds. Holes (S' = 0) and excluded pixels are
+		 * dropped by the consumers, which read the flags anyway. */
 		const int na = sh.na;
 		for (int base = 0; base < na; base += BLOCK) {
+			/* a batch adds at most BLOCK seeds */
+			__syncthreads();
+			if (sh.nseed + BLOCK > SEED_BUF)
+				flush_seeds(sh, p.list_b, seeds);
 			int idx = base + tid;
 			bool seed = false;
 			int i = 0;
@@ -893,25 +929,23 @@ significance(Params p, int iter)
 				int pos = sh.alist[idx], sx = pos % PW, sy = pos / PW;
 				int x = X0 - HP + sx, y = Y0 - HP + sy;
 				i = y * w + x;
-				uint32_t fl = p.flags[fr.off + i];
-				if (!(fl & HOLE)) {
-					int cx = sx + (HS - HP), cy = sy + (HS - HP);
-					double v[25], m;
+				int cx = sx + (HS - HP), cy = sy + (HS - HP);
+				double v[25], m;
 #pragma unroll
-					for (int dy = 0; dy < 5; ++dy)
+				for (int dy = 0; dy < 5; ++dy)
 #pragma unroll
-						for (int dx = 0; dx < 5; ++dx)
-							v[dy * 5 + dx] = sh.s[cy - 2 + dy][cx - 2 + dx];
-					median25(v, &m);
-					double sp = sh.s[cy][cx] - m;
-					p.sprime[fr.off + i] = sp;
-					reinterpret_cast<uint16_t *>(p.flags + fr.off + i)[1] = (uint16_t)stamp;
-					seed = !(fl & EXCL) && sp > t_cr;
-				}
+					for (int dx = 0; dx < 5; ++dx)
+						v[dy * 5 + dx] = sh.s[cy - 2 + dy][cx - 2 + dx];
+				median25(v, &m);
+				double sp = sh.s[cy][cx] - m;
+				p.sprime[fr.off + i] = sp;
+				reinterpret_cast<uint16_t *>(p.flags + fr.off + i)[1] = (uint16_t)stamp;
+				seed = sp > t_cr;
 			}
-			list_push(seed, list_entry(f, i), p.list_b, seeds);
+			lds_append(seed, list_entry(f, i), sh.seeds, &sh.nseed);
 		}
 	}
+	flush_seeds(sh, p.list_b, seeds);
 }
 
 /* F at a seed from the 9 x 9 clean-image block around it (median border
@@ -1002,18 +1036,22 @@ fine(Params p, int iter)
 			entry = p.list_b[e];
 			int f = (int)(entry >> 32), i = (int)(entry & 0xffffffffu);
 			const Frame fr = p.frames[f];
-			double sp = p.sprime[fr.off + i];
-			cand = sp / fine_structure(p, fr, i % fr.w, i / fr.w) > p.contrast;
+			if (!(p.flags[fr.off + i] & EXCL)) {
+				double sp = p.sprime[fr.off + i];
+				cand = sp / fine_structure(p, fr, i % fr.w, i / fr.w) > p.contrast;
+			}
 		}
 		list_push(cand, entry, p.list_c, count);
 	}
 }
 
-/* S' at a pixel if this pass computed it there, else -inf (cannot pass). */
+/* S' at a pixel if this pass computed it there (0 at holes,
+ * src/filters.cuh:137-146), else -inf: neither can pass a threshold t >= 0. */
 __device__ __forceinline__ double
 sprime_at(const Params &p, long long g, unsigned stamp)
 {
-	return (p.flags[g] >> STAMP_SHIFT) == stamp ? p.sprime[g] : -INFINITY;
+	uint32_t fw = p.flags[g];
+	return (fw >> STAMP_SHIFT) == stamp && !(fw & HOLE) ? p.sprime[g] : -INFINITY;
 }
 
 /* First growth (src/filters.cuh:177-193 at t): every pixel next to a
