@@ -132,6 +132,7 @@ def main():
                      "kernel": "detect", "algorithmic_bytes_per_launch": round(per_launch_bytes),
                      "avg_launch_ms": round(detect_ms / launches, 4)},
         "stages_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "lists_per_call": {k: int(profiles[0][k]) for k in ("seeds", "candidates", "grown", "widened")},
         "call_roofline_frac": round(call_bytes / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
     }
 

@@ -60,7 +60,7 @@ class RaggedOptions(ctypes.Structure):
 
 
 PROFILE_FIELDS = ("detect_ms", "replace_ms", "prologue_ms", "epilogue_ms", "detect_launches",
-                  "detect_pixels", "flagged")
+                  "detect_pixels", "flagged", "seeds", "candidates", "grown", "widened")
 
 
 class CusmicError(RuntimeError):

@@ -342,14 +342,14 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 
 	std::vector<int> host(n_todo + 1);
 	std::vector<unsigned long long> work(n_queue);
-	unsigned int nflagged = 0;
+	std::vector<unsigned int> nl(n_nlist);
 	hip_check(hipMemcpyAsync(host.data(), p.todo, n_todo * sizeof(int), hipMemcpyDeviceToHost,
 		stream));
 	hip_check(hipMemcpyAsync(host.data() + n_todo, p.status, sizeof(int), hipMemcpyDeviceToHost,
 		stream));
 	if (profile) {
 		hip_check(hipMemcpyAsync(work.data(), p.work, n_queue * 8, hipMemcpyDeviceToHost, stream));
-		hip_check(hipMemcpyAsync(&nflagged, p.nlist, 4, hipMemcpyDeviceToHost, stream));
+		hip_check(hipMemcpyAsync(nl.data(), p.nlist, n_nlist * 4, hipMemcpyDeviceToHost, stream));
 	}
 	hip_check(hipStreamSynchronize(stream));
 	if (profile) {
@@ -359,7 +359,13 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 		profile[CUSMIC_PROFILE_DETECT_LAUNCHES] = maxiter;
 		for (auto v : work)
 			profile[CUSMIC_PROFILE_DETECT_PIXELS] += double(v);
-		profile[CUSMIC_PROFILE_FLAGGED] = nflagged;
+		profile[CUSMIC_PROFILE_FLAGGED] = nl[0];
+		for (int it = 0; it < maxiter; ++it) {
+			profile[CUSMIC_PROFILE_SEEDS] += nl[2 + maxiter + it];
+			profile[CUSMIC_PROFILE_CANDIDATES] += nl[2 + 2 * maxiter + it];
+			profile[CUSMIC_PROFILE_GROWN] += nl[2 + 3 * maxiter + it];
+			profile[CUSMIC_PROFILE_WIDENED] += nl[2 + 4 * maxiter + it];
+		}
 	}
 	if (host[n_todo] & ERR_NO_DONORS)
 		throw std::invalid_argument("no finite replacement donors");

@@ -213,6 +213,71 @@ def lower_half(n, keep):
     return net, s[:keep], [f"v{i}" for i in range(n)]
 
 
+def median49_steps():
+    """Median of 7 rows of 7 fed one row at a time: step k sorts row k and merges
+    it into the rank-pruned list of the rows before; only the ranks that can
+    still be the median of 49 are kept (at most 22 values live)."""
+    steps = []
+
+    def valid(m):
+        return max(0, m - 25), min(m - 1, 24)
+
+    for k in range(7):
+        net = Net()
+        row = sortnet(net, [net.leaf(f"r{i}") for i in range(7)])
+        if k == 0:
+            out = prune(row, 49)
+            inputs = [f"r{i}" for i in range(7)]
+        else:
+            lo, hi = valid(7 * k)
+            acc = [NINF if i < lo else INF if i > hi else net.leaf(f"a{i - lo}") for i in range(7 * k)]
+            out = prune(merge(net, acc, row), 49)
+            inputs = [f"a{i}" for i in range(hi - lo + 1)] + [f"r{i}" for i in range(7)]
+        lo, hi = valid(7 * (k + 1))
+        steps.append((net, out[lo:hi + 1], inputs))
+    return steps
+
+
+def check_median49_steps(steps, trials=20000, seed=1):
+    rng = np.random.default_rng(seed)
+    for kind in range(3):
+        if kind == 0:
+            x = rng.normal(size=(49, trials))
+        else:
+            x = rng.integers(0, 3 if kind == 1 else 2, size=(49, trials)).astype(float)
+        acc = []
+        for k, (net, outs, inputs) in enumerate(steps):
+            vals = {f"r{i}": x[7 * k + i] for i in range(7)}
+            vals.update({f"a{i}": a for i, a in enumerate(acc)})
+            acc = net.evaluate(outs, vals)
+        assert np.array_equal(np.array(acc[0]), np.sort(x, axis=0)[24]), "median49 steps"
+
+
+def emit_step(name, net, outs, inputs):
+    """Device function of one incremental step: acc (previous valid ranks) and
+    row (7 values) in, new valid ranks out."""
+    names = {}
+    for n in inputs:
+        kind, idx = n[0], int(n[1:])
+        if ("in", n) in net.key:
+            names[net.key[("in", n)]] = f"{'acc' if kind == 'a' else 'row'}[{idx}]"
+    body, tmp = [], 0
+    for node in net.live(outs):
+        d = net.defs[node]
+        if d[0] == "in":
+            continue
+        a = "INFINITY" if d[1] == INF else "-INFINITY" if d[1] == NINF else names[d[1]]
+        b = "INFINITY" if d[2] == INF else "-INFINITY" if d[2] == NINF else names[d[2]]
+        names[node] = f"t{tmp}"
+        body.append(f"\tconst double t{tmp} = {'fmin' if d[0] == 'min' else 'fmax'}({a}, {b});")
+        tmp += 1
+    for i, o in enumerate(outs):
+        body.append(f"\tout[{i}] = {names[o]};")
+    head = (f"/* {len(outs)} ranks out, {tmp} min/max operations */\n"
+            f"static __device__ __forceinline__ void\n{name}(const double *acc, const double *row, double *out)\n{{")
+    return "\n".join([head] + body + ["}"]) + "\n"
+
+
 def check(net, outs, inputs, expect, trials=20000, seed=0):
     rng = np.random.default_rng(seed)
     for kind in range(3):
@@ -278,6 +343,11 @@ def main():
         net, outs, inputs = lower_half(n, keep)
         check(net, outs, inputs, lambda x, keep=keep: list(np.sort(x, axis=0)[:keep]))
         parts.append(emit(f"sort_low{keep}_of{n}", net, outs, inputs, "v", "out"))
+    # incremental median of 49 (fine structure), one 7-value row per step
+    steps = median49_steps()
+    check_median49_steps(steps)
+    for k, (net, outs, inputs) in enumerate(steps):
+        parts.append(emit_step(f"median49_step{k}", net, outs, inputs))
     Path(__file__).with_name("networks.inc").write_text("\n".join(parts))
 
 

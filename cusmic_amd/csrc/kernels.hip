@@ -67,20 +67,51 @@ list_push(bool pred, unsigned long long item, unsigned long long *list, unsigned
 		list[base + __popcll(m & ((1ull << lane) - 1))] = item;
 }
 
-/* Add the number of lanes with pred to per-frame counters, one atomic per
- * distinct frame in the wave (counter of frame f at base + f * stride). */
+/* Per-workgroup output buffer of list entries: lanes append in LDS, then one
+ * global atomic per flush reserves room in the shared list (a single counter
+ * hammered by every wave serialises in L2). */
+template <int CAP>
+struct BlockList {
+	unsigned long long item[CAP];
+	int n;
+	unsigned base;
+};
+
+template <int CAP>
 __device__ __forceinline__ void
-wave_frame_add(bool pred, int f, int *base, int stride, int sign)
+block_list_flush(BlockList<CAP> &b, unsigned long long *list, unsigned int *count)
 {
-	unsigned long long m = __ballot(pred);
+	__syncthreads();
+	const int n = b.n;
+	if (!threadIdx.x && n)
+		b.base = atomicAdd(count, (unsigned)n);
+	__syncthreads();
+	for (int k = threadIdx.x; k < n; k += BLOCK)
+		list[b.base + k] = b.item[k];
+	__syncthreads();
+	if (!threadIdx.x)
+		b.n = 0;
+	__syncthreads();
+}
+
+/* Add each lane's value to the counter of its frame (base + f * stride), one
+ * atomic per distinct frame in the wave. */
+__device__ __forceinline__ void
+wave_frame_sum(int value, int f, int *base, int stride)
+{
+	unsigned long long m = __ballot(value != 0);
 	int lane = threadIdx.x & 63;
 	while (m) {
 		int leader = __ffsll((long long)m) - 1;
 		int lf = __shfl(f, leader, 64);
-		unsigned long long same = __ballot(pred && f == lf);
+		bool same = value != 0 && f == lf;
+		unsigned long long sm = __ballot(same);
+		int v = same ? value : 0;
+		for (int o = 32; o; o >>= 1)
+			v += __shfl_xor(v, o, 64);
 		if (lane == leader)
-			atomicAdd(base + (size_t)lf * stride, sign * __popcll(same));
-		m &= ~same;
+			atomicAdd(base + (size_t)lf * stride, v);
+		m &= ~sm;
 	}
 }
 
@@ -948,64 +979,98 @@ significance(Params p, int iter)
 	flush_seeds(sh, p.list_b, seeds);
 }
 
-/* F at a seed from the 9 x 9 clean-image block around it (median border
- * mode): 49 med3 values, their med7, and N (src/filters.cuh:148-162). */
+__device__ __forceinline__ double
+med3(double a, double b, double c)
+{
+	return fmax(fmin(a, b), fmin(fmax(a, b), c));
+}
+
+/* Feed row j of the 7 x 7 med3 block to the incremental median of 49. */
+__device__ __forceinline__ void
+median49_feed(int j, const double *acc, const double *row, double *out)
+{
+	switch (j) {
+	case 0: median49_step0(acc, row, out); break;
+	case 1: median49_step1(acc, row, out); break;
+	case 2: median49_step2(acc, row, out); break;
+	case 3: median49_step3(acc, row, out); break;
+	case 4: median49_step4(acc, row, out); break;
+	case 5: median49_step5(acc, row, out); break;
+	default: median49_step6(acc, row, out); break;
+	}
+}
+
+/* F at a seed (src/filters.cuh:148-162): m3 = med3 of the clean image (median
+ * border mode) over the 7 x 7 positions around it, med7 = their median, fed row
+ * by row so that at most 22 ranks stay live, and N at the seed. */
 __device__ double
 fine_structure(const Params &p, const Frame &fr, int x, int y)
 {
 	const double *clean = p.clean + fr.off;
 	const int w = fr.w, h = fr.h, mode = p.mode;
-	double m3[49];
 	const bool inside = x >= 3 && x + 3 < w && y >= 3 && y + 3 < h;
-	if (inside || mode != CUSMIC_NEAREST) {
-		/* in the extended image of a symmetric or periodic mode, med3 at an
-		 * outside position equals med3 at its mapped position; CONSTANT
-		 * positions outside the frame are 0. Rows of the 9 x 9 block roll
-		 * through three registers sets. */
-		double a[9], b[9], c[9];
+	/* in the extended image of a symmetric or periodic mode, med3 at an outside
+	 * position equals med3 at its mapped position; CONSTANT positions outside
+	 * the frame are 0; NEAREST near the edge takes the clamped position */
+	const bool block = inside || mode != CUSMIC_NEAREST;
+	double acc[2][22], a[9], b[9], c[9], centre = 0.0;
+	if (block) {
 #pragma unroll
 		for (int i = 0; i < 9; ++i) {
 			a[i] = med_sample(clean, x - 4 + i, y - 4, w, h, mode);
 			b[i] = med_sample(clean, x - 4 + i, y - 3, w, h, mode);
 		}
+	}
 #pragma unroll
-		for (int j = 0; j < 7; ++j) {
+	for (int j = 0; j < 7; ++j) {
+		double row[7];
+		if (block) {
 #pragma unroll
 			for (int i = 0; i < 9; ++i)
 				c[i] = med_sample(clean, x - 4 + i, y - 2 + j, w, h, mode);
-			/* keep the loads of later rows from being hoisted here */
-			__builtin_amdgcn_sched_barrier(0);
+			/* 3x3 medians from sorted columns: med9 = med3(max of minima,
+			 * med3 of middles, min of maxima) */
+			double lo[9], md[9], hi[9];
+#pragma unroll
+			for (int i = 0; i < 9; ++i) {
+				double l1 = fmin(a[i], b[i]), h1 = fmax(a[i], b[i]);
+				double t = fmax(l1, c[i]);
+				lo[i] = fmin(l1, c[i]);
+				md[i] = fmin(h1, t);
+				hi[i] = fmax(h1, t);
+			}
 #pragma unroll
 			for (int i = 0; i < 7; ++i) {
-				double v9[9] = {a[i], a[i + 1], a[i + 2], b[i], b[i + 1], b[i + 2],
-						c[i], c[i + 1], c[i + 2]};
-				double m;
-				median9(v9, &m);
+				double L = fmax(fmax(lo[i], lo[i + 1]), lo[i + 2]);
+				double H = fmin(fmin(hi[i], hi[i + 1]), hi[i + 2]);
+				double M = med3(md[i], md[i + 1], md[i + 2]);
+				double m = med3(L, M, H);
 				int qx = x - 3 + i, qy = y - 3 + j;
 				bool out = qx < 0 || qx >= w || qy < 0 || qy >= h;
-				m3[j * 7 + i] = mode == CUSMIC_CONSTANT && out ? 0.0 : m;
+				row[i] = mode == CUSMIC_CONSTANT && out ? 0.0 : m;
 			}
 #pragma unroll
 			for (int i = 0; i < 9; ++i) {
 				a[i] = b[i];
 				b[i] = c[i];
 			}
-		}
-	} else {
-		/* NEAREST near the edge: med3 of the clamped position */
-		for (int j = 0; j < 7; ++j)
+		} else {
 			for (int i = 0; i < 7; ++i) {
 				int qx = min(max(x - 3 + i, 0), w - 1), qy = min(max(y - 3 + j, 0), h - 1);
-				double v9[9], m;
-				for (int a = 0; a < 3; ++a)
-					for (int c = 0; c < 3; ++c)
-						v9[a * 3 + c] = med_sample(clean, qx - 1 + c, qy - 1 + a, w, h, mode);
-				median9(v9, &m);
-				m3[j * 7 + i] = m;
+				double v9[9];
+				for (int r = 0; r < 3; ++r)
+					for (int q = 0; q < 3; ++q)
+						v9[r * 3 + q] = med_sample(clean, qx - 1 + q, qy - 1 + r, w, h, mode);
+				median9(v9, &row[i]);
 			}
+		}
+		if (j == 3)
+			centre = row[3];
+		median49_feed(j, acc[(j + 1) & 1], row, acc[j & 1]);
+		/* keep the loads of later rows from being hoisted here */
+		__builtin_amdgcn_sched_barrier(0);
 	}
-	double med7;
-	median49(m3, &med7);
+	const double med7 = acc[0][0];
 	double nz;
 	if (fr.error) {
 		nz = fr.error[y * w + x];
@@ -1019,15 +1084,18 @@ fine_structure(const Params &p, const Frame &fr, int x, int y)
 		double rn = fr.readnoise ? fr.readnoise[y * w + x] : fr.rn;
 		nz = noise_model(m5, g, rn);
 	}
-	double fine = (m3[24] - med7) / nz;
+	double fine = (centre - med7) / nz;
 	return fine < FINE_FLOOR ? FINE_FLOOR : fine;
 }
 
 __global__ __launch_bounds__(BLOCK) void
 fine(Params p, int iter)
 {
+	__shared__ BlockList<BLOCK> out;
 	const unsigned n = p.nlist[nl_seed(p, iter)];
 	unsigned int *count = &p.nlist[nl_cand(p, iter)];
+	if (!threadIdx.x)
+		out.n = 0;
 	for (unsigned base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
 		unsigned e = base + threadIdx.x;
 		bool cand = false;
@@ -1041,7 +1109,9 @@ fine(Params p, int iter)
 				cand = sp / fine_structure(p, fr, i % fr.w, i / fr.w) > p.contrast;
 			}
 		}
-		list_push(cand, entry, p.list_c, count);
+		__syncthreads();
+		lds_append(cand, entry, out.item, &out.n);
+		block_list_flush(out, p.list_c, count);
 	}
 }
 
@@ -1059,9 +1129,12 @@ sprime_at(const Params &p, long long g, unsigned stamp)
 __global__ __launch_bounds__(BLOCK) void
 grow_first(Params p, int iter)
 {
+	__shared__ BlockList<BLOCK * 9> out;
 	const unsigned n = p.nlist[nl_cand(p, iter)];
 	const unsigned stamp = pass_stamp(iter);
 	unsigned int *count = &p.nlist[nl_grow(p, iter)];
+	if (!threadIdx.x)
+		out.n = 0;
 	for (unsigned base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
 		unsigned e = base + threadIdx.x;
 		int f = 0, x = 0, y = 0, w = 1, h = 1;
@@ -1077,14 +1150,16 @@ grow_first(Params p, int iter)
 			x = i % w;
 			y = i / w;
 		}
+		__syncthreads();
 		for (int k = 0; k < 9; ++k) {
 			int qx = x + k % 3 - 1, qy = y + k / 3 - 1;
 			bool join = false;
 			int q = qy * w + qx;
 			if (e < n && qx >= 0 && qx < w && qy >= 0 && qy < h && sprime_at(p, off + q, stamp) > p.threshold)
 				join = !(atomicOr(&p.flags[off + q], GROW) & GROW);
-			list_push(join, list_entry(f, q), p.list_b, count);
+			lds_append(join, list_entry(f, q), out.item, &out.n);
 		}
+		block_list_flush(out, p.list_b, count);
 	}
 }
 
@@ -1094,11 +1169,14 @@ grow_first(Params p, int iter)
 __global__ __launch_bounds__(BLOCK) void
 grow_second(Params p, int iter)
 {
+	__shared__ BlockList<BLOCK * 9> out;
 	const unsigned n = p.nlist[nl_grow(p, iter)];
 	const unsigned stamp = pass_stamp(iter);
+	if (!threadIdx.x)
+		out.n = 0;
 	for (unsigned base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
 		unsigned e = base + threadIdx.x;
-		int f = 0, x = 0, y = 0, w = 1, h = 1;
+		int f = 0, x = 0, y = 0, w = 1, h = 1, fresh_n = 0, lost_n = 0;
 		long long off = 0;
 		if (e < n) {
 			unsigned long long entry = p.list_b[e];
@@ -1112,22 +1190,25 @@ grow_second(Params p, int iter)
 			y = i / w;
 			atomicAnd(&p.flags[off + i], ~GROW);
 		}
+		__syncthreads();
 		for (int k = 0; k < 9; ++k) {
 			int qx = x + k % 3 - 1, qy = y + k / 3 - 1;
-			bool fresh = false, lost = false;
+			bool fresh = false;
 			int q = qy * w + qx;
 			if (e < n && qx >= 0 && qx < w && qy >= 0 && qy < h &&
 			    sprime_at(p, off + q, stamp) > p.neighbor) {
 				uint32_t old = atomicOr(&p.flags[off + q], CRAY);
 				if (!(old & CRAY)) {
 					fresh = true;
-					lost = !(old & EXCL) && isfinite(p.clean[off + q]);
+					fresh_n += 1;
+					lost_n += !(old & EXCL) && isfinite(p.clean[off + q]);
 				}
 			}
-			list_push(fresh, list_entry(f, q), p.list, &p.nlist[0]);
-			wave_frame_add(fresh, f, p.todo + iter * p.nf, 1, 1);
-			wave_frame_add(lost, f, &p.counts[0].donors, int(sizeof(Counts) / sizeof(int)), -1);
+			lds_append(fresh, list_entry(f, q), out.item, &out.n);
 		}
+		wave_frame_sum(fresh_n, f, p.todo + iter * p.nf, 1);
+		wave_frame_sum(-lost_n, f, &p.counts[0].donors, int(sizeof(Counts) / sizeof(int)));
+		block_list_flush(out, p.list, &p.nlist[0]);
 	}
 }
 

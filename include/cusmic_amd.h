@@ -38,7 +38,8 @@ struct cusmic_ragged_options {
 enum cusmic_profile {
 	CUSMIC_PROFILE_DETECT_MS, CUSMIC_PROFILE_REPLACE_MS, CUSMIC_PROFILE_PROLOGUE_MS,
 	CUSMIC_PROFILE_EPILOGUE_MS, CUSMIC_PROFILE_DETECT_LAUNCHES, CUSMIC_PROFILE_DETECT_PIXELS,
-	CUSMIC_PROFILE_FLAGGED, CUSMIC_PROFILE_SIZE
+	CUSMIC_PROFILE_FLAGGED, CUSMIC_PROFILE_SEEDS, CUSMIC_PROFILE_CANDIDATES,
+	CUSMIC_PROFILE_GROWN, CUSMIC_PROFILE_WIDENED, CUSMIC_PROFILE_SIZE
 };
 
 /* Ragged batch: 1..65535 contiguous device frames of individual shapes,
