@@ -14,14 +14,21 @@
 
 namespace cusmic {
 
-/* Detection tile: 56 x 32 outputs, so that the significance region with its
- * 4-pixel halo is exactly one wave (64 lanes) wide. */
-constexpr int TILE_W = 56;
+/* Detection tile: 60 x 32 outputs, so that the significance region with the
+ * 2-pixel halo of the 5x5 median is exactly one wave (64 lanes) wide. */
+constexpr int TILE_W = 60;
 constexpr int TILE_H = 32;
 constexpr int BLOCK = 256;
 
+/* The first detection pass walks each frame in runs of up to RUN_TILES
+ * vertically adjacent tiles, so that the significance rows shared by two
+ * tiles of a run are computed once. */
+constexpr int RUN_TILES = 8;
+
 /* Per-pixel state word (u32) of the flags plane: bits 0-7 are flags, bits
- * 16-31 hold the stamp of the last pass that computed S' at the pixel. */
+ * 16-31 hold the stamp of the last pass that computed S' at the pixel. S' at a
+ * pixel is current iff that stamp equals the stamp of the last pass that
+ * processed the pixel's tile (Params::tile_pass). */
 constexpr uint32_t HOLE = 1;    /* input pixel is nonfinite */
 constexpr uint32_t EXCL = 2;    /* hole or masked: never a seed or donor */
 constexpr uint32_t CRAY = 4;    /* flagged as cosmic ray (the output mask) */
@@ -45,6 +52,7 @@ struct Frame {
 	int w, h;
 	int tile0, tiles_x, tiles_y;
 	int chunk0;            /* first CHUNK-pixel chunk of the frame */
+	int run0;              /* first first-pass run of the frame */
 };
 
 /* Pointwise passes stream frames in chunks of CHUNK consecutive pixels. */
@@ -76,12 +84,25 @@ struct Params {
 					  * [2 + 4M + iter] widened replacements (iter = M: fill) */
 	unsigned int *dirty;             /* [2][ntiles] dirty flags */
 	unsigned int *dirty_list;        /* [2][ntiles] dirty tile lists */
-	int *status;                     /* [0] error bits */
+	int *status;                     /* [0] error bits, [1] nonzero if a clean
+					  * sample is infinite (D + bg overflowed) */
 	unsigned long long *work;        /* [maxiter] in-frame pixels of processed tiles */
 	int *tile_frame;                 /* [ntiles] frame of each tile */
 	int *chunk_frame;                /* [nchunks] frame of each chunk */
 	int nchunks;
+	int *run_frame;                  /* [nruns] frame of each first-pass run */
+	int nruns;
+	unsigned int *tile_pass;         /* [ntiles] stamp of the last pass that
+					  * processed the tile */
 };
+
+/* Tiles per run, in run order: frame-major, then row-major over blocks of
+ * RUN_TILES tile rows. */
+__device__ __host__ __forceinline__ int
+runs_y(int tiles_y)
+{
+	return (tiles_y + RUN_TILES - 1) / RUN_TILES;
+}
 
 constexpr int ERR_NO_DONORS = 1;
 

@@ -116,39 +116,6 @@ wave_frame_sum(int value, int f, int *base, int stride)
 }
 
 __device__ __forceinline__ double
-positive(double v)
-{
-	return v < 0 ? 0.0 : v;
-}
-
-/* Laplacian of the 2x-replicated frame (src/filters.cuh:86-105).
- * c, u, d, l, r are the pixel and its up/down/left/right samples; exact is the
- * L.A.Cosmic (scipy) accumulation order, otherwise the reference CUDA order;
- * narrow marks a one-pixel-wide frame, whose 2x2 block sum numpy runs
- * sequentially. */
-__device__ __forceinline__ double
-laplacian(double c, double u, double d, double l, double r, bool exact, bool narrow)
-{
-	c = c / 4;
-	u = u / 4;
-	d = d / 4;
-	l = l / 4;
-	r = r / 4;
-	if (!exact) {
-		double up = c - u, down = c - d, left = c - l, right = c - r;
-		return (positive(up + left) + positive(up + right)) +
-		       (positive(down + left) + positive(down + right));
-	}
-	double ul = positive(((((0.0 - u) - l) + 4 * c) - c) - c);
-	double ur = positive(((((0.0 - u) - c) + 4 * c) - r) - c);
-	double dl = positive(((((0.0 - c) - l) + 4 * c) - c) - d);
-	double dr = positive(((((0.0 - c) - c) + 4 * c) - r) - d);
-	if (narrow)
-		return ((ul + ur) + dl) + dr;
-	return (ul + ur) + (dl + dr);
-}
-
-__device__ __forceinline__ double
 noise_model(double med5, double g, double rn)
 {
 	/* src/filters.cuh:107-123 */
@@ -184,25 +151,27 @@ tile_of(const Frame &fr, int tile)
 	return t;
 }
 
-/* tile -> frame and chunk -> frame tables (binary search once per entry) */
+/* tile -> frame, chunk -> frame and run -> frame tables (binary search once
+ * per entry) */
 __global__ void
 set_tables(Params p)
 {
-	for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < p.ntiles + p.nchunks;
+	for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < p.ntiles + p.nchunks + p.nruns;
 	     t += gridDim.x * blockDim.x) {
 		if (t < p.ntiles) {
 			p.tile_frame[t] = find_frame(p.frames, p.nf, t);
-		} else {
-			int c = t - p.ntiles, lo = 0, hi = p.nf - 1;
-			while (lo < hi) {
-				int mid = (lo + hi + 1) >> 1;
-				if (p.frames[mid].chunk0 <= c)
-					lo = mid;
-				else
-					hi = mid - 1;
-			}
-			p.chunk_frame[c] = lo;
+			continue;
 		}
+		const bool chunk = t < p.ntiles + p.nchunks;
+		int c = chunk ? t - p.ntiles : t - p.ntiles - p.nchunks, lo = 0, hi = p.nf - 1;
+		while (lo < hi) {
+			int mid = (lo + hi + 1) >> 1;
+			if ((chunk ? p.frames[mid].chunk0 : p.frames[mid].run0) <= c)
+				lo = mid;
+			else
+				hi = mid - 1;
+		}
+		(chunk ? p.chunk_frame : p.run_frame)[c] = lo;
 	}
 }
 
@@ -223,6 +192,7 @@ prologue(Params p)
 	const int n = fr.w * fr.h, start = (blockIdx.x - fr.chunk0) * CHUNK + threadIdx.x;
 	const bool bg_map = fr.background != nullptr, bg = bg_map || !isnan(p.bg);
 	int holes = 0, allowed = 0, donors = 0;
+	bool infinite = false;
 	for (int base = start; base < start + CHUNK; base += PER * BLOCK) {
 		double d[PER], b[PER];
 		uint8_t m[PER];
@@ -254,11 +224,16 @@ prologue(Params p)
 				holes += hole;
 				allowed += allow;
 				donors += allow && isfinite(c);
+				infinite |= isinf(c);
 			}
 			/* hole list (only consulted when maxiter > 0) */
 			list_push(hole, list_entry(f, i), p.list, &p.nlist[1]);
 		}
 	}
+	/* an infinite clean sample (D + bg overflowed) can make the Laplacian
+	 * NaN, which the detection then has to keep */
+	if (__ballot(infinite) && !(threadIdx.x & 63))
+		atomicOr(&p.status[1], 1);
 	holes = block_sum(holes, red);
 	allowed = block_sum(allowed, red);
 	donors = block_sum(donors, red);
@@ -699,31 +674,79 @@ replace_widened(Params p, int iter, int fill)
 /* Detection (src/filters.cuh:86-193, src/cleaner.cuh:22-37,102-126)         */
 /*
  * One detection pass is four launches:
- *   significance  dense, per 56 x 32 tile: S = L / (2 N) with a 4-pixel halo
- *                 in LDS; S' = S - med5(S) only where S > min(t, t_n) (S' <= S
- *                 because S >= 0); S' is written to a sparse plane stamped
- *                 with the pass; seeds (S' > t, not excluded) go to a list
+ *   significance  dense: S = L / (2 N) over each tile and a 2-pixel halo, kept
+ *                 in an LDS ring of rows; S' = S - med5(S) only where
+ *                 S > min(t, t_n) (S' <= S because S >= 0); S' is written to a
+ *                 sparse plane stamped with the pass; seeds (S' > t) go to a
+ *                 list. The first pass walks runs of RUN_TILES tiles down a
+ *                 tile column, so rows shared by two tiles are computed once;
+ *                 later passes process the tiles marked dirty.
  *   fine          per seed: F = max((m3 - med7(m3)) / N, 0.01), m3 = med3(C);
  *                 S' / F > contrast puts the seed on the candidate list
  *   grow_first    per candidate: neighbours with S' > t join the growth list
  *   grow_second   per grown pixel: neighbours with S' > t_n are flagged; a
  *                 pixel flagged for the first time is new (accumulate)
- * Every list entry is a correct member of its set, and a pixel outside the
- * processed tiles cannot be new, so the sparse passes give the reference's
- * dense result exactly.
+ * A tile that a pass skips had no change of the clean image within REACH of
+ * its pixels since the pass that last processed it, so its S' values and the
+ * growth of its candidates are still those of that pass: the sparse passes
+ * give the reference's dense result exactly.
  */
 
-constexpr int HS = 4, HP = 2;
-constexpr int SW = TILE_W + 2 * HS, SH = TILE_H + 2 * HS;  /* 64 x 40 */
-constexpr int PW = TILE_W + 2 * HP, PH = TILE_H + 2 * HP;  /* 60 x 36 */
-constexpr int ROWS_PER_WAVE = SH / 4;                      /* 10 */
+constexpr int HS = 2;                      /* halo of S: the 5x5 median */
+constexpr int SW = TILE_W + 2 * HS;        /* 64: one wave */
+constexpr int STEP_ROWS = TILE_H + 2 * HS; /* rows of S the medians of one tile read */
+constexpr int RING = 40;                   /* rows of S kept in LDS */
 static_assert(SW == 64, "the significance region is one wave wide");
-static_assert(SH % 4 == 0, "four waves share the significance rows");
+static_assert(STEP_ROWS % 4 == 0 && TILE_H % 4 == 0, "four waves share the rows of a step");
+static_assert(RING >= STEP_ROWS && RING <= 64, "the ring holds the rows of one tile");
 
 __device__ __forceinline__ unsigned
 pass_stamp(int iter)
 {
 	return unsigned(iter % 65535) + 1;
+}
+
+/* max(v, 0) as the reference writes it (v < 0 ? 0 : v), which keeps a NaN.
+ * Without infinite samples (TAME) no NaN can arise and one max instruction
+ * does; the two differ at most in the sign of a zero, which no threshold test
+ * sees. */
+template <bool TAME>
+__device__ __forceinline__ double
+clip(double v)
+{
+	return TAME ? fmax(v, 0.0) : (v < 0 ? 0.0 : v);
+}
+
+/* Laplacian of the 2x-replicated frame (src/filters.cuh:86-105) from the
+ * quarter samples (x / 4) of the pixel and of its up, down, left and right
+ * neighbours. EXACT is the L.A.Cosmic (scipy) accumulation order, otherwise
+ * the reference CUDA order; narrow marks a one-pixel-wide frame, whose 2x2
+ * block sum numpy runs sequentially. A negated first operand stands for the
+ * leading 0 - x of the scipy sum: that changes only the sign of a zero. */
+template <bool EXACT, bool TAME>
+__device__ __forceinline__ double
+laplacian_q(double c, double u, double d, double l, double r, bool narrow)
+{
+	if (!EXACT) {
+		double up = c - u, down = c - d, left = c - l, right = c - r;
+		return (clip<TAME>(up + left) + clip<TAME>(up + right)) +
+		       (clip<TAME>(down + left) + clip<TAME>(down + right));
+	}
+	double c4 = 4 * c;
+	double ul = clip<TAME>((((-u - l) + c4) - c) - c);
+	double ur = clip<TAME>((((-u - c) + c4) - r) - c);
+	double dl = clip<TAME>((((-c - l) + c4) - c) - d);
+	double dr = clip<TAME>((((-c - c) + c4) - r) - d);
+	if (narrow)
+		return ((ul + ur) + dl) + dr;
+	return (ul + ur) + (dl + dr);
+}
+
+/* A noise value that breaks S >= 0 (zero or negative; NaN only makes S NaN). */
+__device__ __forceinline__ bool
+invalid_noise(double n)
+{
+	return !(n > 0) && !isnan(n);
 }
 
 /* Laplacian sample of the clean image at (x, y), at most one step outside the
@@ -769,14 +792,15 @@ map_index(int x, int y, int w, int h, int mode)
 	return y * w + x;
 }
 
-
 constexpr int SEED_BUF = 512; /* seeds buffered per workgroup between flushes */
 
 struct SigShared {
-	double s[SH][SW];
-	uint16_t alist[PW * PH];
+	double s[RING][SW];              /* S, row r of a run in slot r % RING */
+	unsigned long long screen[RING]; /* per row: tile columns with S > min(t, t_n) */
+	int bad[RING];                   /* per row: an invalid noise value */
+	uint16_t alist[TILE_W * TILE_H]; /* screened pixels of the tile: row << 6 | lane */
 	unsigned long long seeds[SEED_BUF];
-	int na, tile, unsafe, nseed;
+	int na, nseed, item;
 	unsigned flush_base;
 };
 
@@ -796,187 +820,293 @@ flush_seeds(SigShared &sh, unsigned long long *list, unsigned int *count)
 		sh.nseed = 0;
 }
 
-template <bool MODEL>
-__global__ __launch_bounds__(BLOCK) void
-significance(Params p, int iter)
+/* Store one row of S (this lane's slot) and its screen and noise flags. */
+__device__ __forceinline__ void
+emit_row(SigShared &sh, int &slot, int lane, double t_min, unsigned long long cols, double v, bool bad)
 {
-	__shared__ SigShared sh;
-	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	sh.s[slot][lane] = v;
+	unsigned long long m = __ballot(v > t_min) & cols;
+	bool any_bad = __ballot(bad) != 0;
+	if (!lane) {
+		sh.screen[slot] = m;
+		sh.bad[slot] = any_bad;
+	}
+	slot = slot + 1 == RING ? 0 : slot + 1;
+}
+
+/* One row of S under the noise model, whose 5x5 median of C is med. */
+template <bool EXACT, bool TAME>
+__device__ __forceinline__ void
+model_row(SigShared &sh, int &slot, int lane, double t_min, unsigned long long cols, const Params &p,
+	const Frame &fr, const double *clean, int gx, int gy, double med)
+{
+	const int mode = p.mode, w = fr.w, h = fr.h;
+	double qc = lap_sample(clean, gx, gy, w, h, mode) * 0.25;
+	double qu = lap_sample(clean, gx, gy - 1, w, h, mode) * 0.25;
+	double qd = lap_sample(clean, gx, gy + 1, w, h, mode) * 0.25;
+	double ql = lap_sample(clean, gx - 1, gy, w, h, mode) * 0.25;
+	double qr = lap_sample(clean, gx + 1, gy, w, h, mode) * 0.25;
+	int mi = map_index(gx, gy, w, h, mode);
+	double gg = fr.gain ? fr.gain[mi] : fr.g;
+	double rn = fr.readnoise ? fr.readnoise[mi] : fr.rn;
+	double nz = noise_model(med, gg, rn);
+	bool in = gx >= 0 && gx < w && gy >= 0 && gy < h;
+	emit_row(sh, slot, lane, t_min, cols, laplacian_q<EXACT, TAME>(qc, qu, qd, ql, qr, w == 1) / (2 * nz),
+		(in || mode == CUSMIC_WRAP) && invalid_noise(nz));
+}
+
+/* One wave: S of run rows [r0, r0 + NR) into the ring (run row r is frame row
+ * yb + r, lane l is column X0 - HS + l), with each row's screened columns
+ * (among cols) and whether it holds an invalid noise value. Slots outside the
+ * frame are computed from in-bounds samples; the caller overwrites the ones
+ * the medians read, unless the mode is WRAP, under which every stencil
+ * commutes with the periodic extension. NR is a constant so that the row loop
+ * unrolls around its ballots and the loads of several rows are in flight. */
+template <bool MODEL, bool EXACT, bool TAME, int NR>
+__device__ __forceinline__ void
+significance_rows(SigShared &sh, const Params &p, const Frame &fr, int X0, int yb, int r0,
+	double t_min, unsigned long long cols)
+{
+	/* an opaque lane keeps the compiler from hoisting this step's per-lane
+	 * addresses out of the step loop, where they would stay live across the
+	 * medians */
+	int lane = threadIdx.x & 63;
+	asm volatile("" : "+v"(lane));
+	const int mode = p.mode, w = fr.w, h = fr.h, nr = NR;
+	const bool wrap = mode == CUSMIC_WRAP, narrow = w == 1;
+	const double *clean = p.clean + fr.off;
+	const int gx = X0 - HS + lane, ya = yb + r0;
+	int slot = r0 % RING;
+	/* the rows and their one-pixel ring lie inside the frame */
+	const bool inner = X0 - HS - 1 >= 0 && X0 - HS + SW < w && ya - 1 >= 0 && ya + nr < h;
+	if ((!MODEL || fr.error) && inner) {
+		const double *rp = clean + (long long)ya * w + gx;
+		const double *ep = fr.error + (long long)ya * w + gx;
+		double qu = rp[-w] * 0.25, qc = rp[0] * 0.25;
+		/* convergent ballots allow only an unroll factor that divides NR */
+		constexpr int UNROLL = NR % 4 == 0 ? 4 : NR % 3 == 0 ? 3 : 1;
+#pragma unroll UNROLL
+		for (int r = 0; r < nr; ++r) {
+			double dn = rp[w], l = rp[-1], rt = rp[1], nz = *ep;
+			double qd = dn * 0.25;
+			emit_row(sh, slot, lane, t_min, cols, laplacian_q<EXACT, TAME>(qc, qu, qd, l * 0.25, rt * 0.25, false) / (2 * nz),
+				invalid_noise(nz));
+			qu = qc;
+			qc = qd;
+			rp += w;
+			ep += w;
+		}
+	} else if (!MODEL || fr.error) {
+		double qu = lap_sample(clean, gx, ya - 1, w, h, mode) * 0.25;
+		double qc = lap_sample(clean, gx, ya, w, h, mode) * 0.25;
+#pragma unroll 1
+		for (int r = 0; r < nr; ++r) {
+			int gy = ya + r;
+			double qd = lap_sample(clean, gx, gy + 1, w, h, mode) * 0.25;
+			double ql = lap_sample(clean, gx - 1, gy, w, h, mode) * 0.25;
+			double qr = lap_sample(clean, gx + 1, gy, w, h, mode) * 0.25;
+			double nz = fr.error[map_index(gx, gy, w, h, mode)];
+			bool in = gx >= 0 && gx < w && gy >= 0 && gy < h;
+			emit_row(sh, slot, lane, t_min, cols, laplacian_q<EXACT, TAME>(qc, qu, qd, ql, qr, narrow) / (2 * nz),
+				(in || wrap) && invalid_noise(nz));
+			qu = qc;
+			qc = qd;
+		}
+	} else {
+		/* noise model: N from strip-shared 5x5 medians, four rows at a time */
+#pragma unroll 1
+		for (int g = 0; g < nr; g += 4) {
+			int gy0 = ya + g;
+			double rows[8 * 5], med[4];
+#pragma unroll
+			for (int j = 0; j < 8; ++j)
+#pragma unroll
+				for (int c = 0; c < 5; ++c)
+					rows[j * 5 + c] = med_sample(clean, gx - 2 + c, gy0 - 2 + j, w, h, mode);
+			strip_median5x4(rows, med);
+			const int n_out = min(4, nr - g);
+			model_row<EXACT, TAME>(sh, slot, lane, t_min, cols, p, fr, clean, gx, gy0, med[0]);
+			if (n_out > 1)
+				model_row<EXACT, TAME>(sh, slot, lane, t_min, cols, p, fr, clean, gx, gy0 + 1, med[1]);
+			if (n_out > 2)
+				model_row<EXACT, TAME>(sh, slot, lane, t_min, cols, p, fr, clean, gx, gy0 + 2, med[2]);
+			if (n_out > 3)
+				model_row<EXACT, TAME>(sh, slot, lane, t_min, cols, p, fr, clean, gx, gy0 + 3, med[3]);
+		}
+	}
+}
+
+template <bool MODEL, bool EXACT, bool TAME>
+__device__ __forceinline__ void
+significance_pass(SigShared &sh, const Params &p, int iter)
+{
+	const int wave = threadIdx.x >> 6;
 	const int mode = p.mode;
 	const bool wrap = mode == CUSMIC_WRAP;
-	const bool exact = p.laplacian == 0;
 	const double t_cr = p.threshold, t_min = fmin(p.threshold, p.neighbor);
 	const unsigned stamp = pass_stamp(iter);
-	const unsigned limit = iter == 0 ? (unsigned)p.ntiles : p.nlist[nl_dirty(p, iter)];
+	const unsigned limit = iter == 0 ? (unsigned)p.nruns : p.nlist[nl_dirty(p, iter)];
 	const unsigned int *dlist = p.dirty_list + (size_t)(iter & 1) * p.ntiles;
 	unsigned int *seeds = &p.nlist[nl_seed(p, iter)];
 	unsigned next = 0;
-	if (!tid) {
+	unsigned long long work = 0; /* in-frame pixels processed, thread 0 */
+	if (!threadIdx.x) {
 		next = atomicAdd(&p.queue[iter], 1u);
 		sh.nseed = 0;
 	}
 
 	for (;;) {
 		__syncthreads();
-		if (!tid) {
+		if (!threadIdx.x) {
 			unsigned q = next;
-			sh.tile = q < limit ? (iter == 0 ? (int)q : (int)dlist[q]) : -1;
-			sh.na = sh.unsafe = 0;
+			sh.item = q < limit ? (iter == 0 ? (int)q : (int)dlist[q]) : -1;
 			if (q < limit)
 				next = atomicAdd(&p.queue[iter], 1u);
 		}
 		__syncthreads();
-		const int tile = sh.tile;
-		if (tile < 0)
+		const int item = __builtin_amdgcn_readfirstlane(sh.item);
+		if (item < 0)
 			break;
-		if (iter > 0 && !tid)
-			p.dirty[(size_t)(iter & 1) * p.ntiles + tile] = 0;
-		const int f = p.tile_frame[tile];
+		/* the run: tile column tx, tile rows [ty0, ty0 + K) of frame f */
+		const int f = iter == 0 ? p.run_frame[item] : p.tile_frame[item];
 		const Frame fr = p.frames[f];
-		const TilePixels t = tile_of(fr, tile);
-		const int w = t.w, h = t.h, X0 = t.x0, Y0 = t.y0;
-		const double *clean = p.clean + fr.off;
-		const bool narrow = w == 1;
-		if (!tid)
-			atomicAdd(&p.work[iter], (unsigned long long)(min(TILE_W, w - X0) * min(TILE_H, h - Y0)));
-
-		/* 1. S over the 64 x 40 region: lane = column, each wave streams ten
-		 * rows. Slots outside the frame are computed from in-bounds samples
-		 * and replaced in step 2 unless the mode is WRAP, under which every
-		 * stencil commutes with the periodic extension. */
-		const int gx = X0 - HS + lane;
-		const int r0 = wave * ROWS_PER_WAVE;
-		bool bad = false;
-		/* the region and its one-pixel ring lie inside the frame */
-		const bool inner = X0 - HS - 1 >= 0 && X0 - HS + SW < w && Y0 - HS - 1 >= 0 && Y0 - HS + SH < h;
-		if ((!MODEL || fr.error) && inner) {
-			/* interior tile: direct row pointers, no border logic */
-			const double *rp = clean + (long long)(Y0 - HS + r0) * w + gx;
-			const double *ep = fr.error + (long long)(Y0 - HS + r0) * w + gx;
-			double up = rp[-w], cur = rp[0];
-#pragma unroll 5
-			for (int r = 0; r < ROWS_PER_WAVE; ++r) {
-				double dn = rp[w], l = rp[-1], rt = rp[1], nz = *ep;
-				bad |= !(nz > 0) && !isnan(nz);
-				sh.s[r0 + r][lane] = laplacian(cur, up, dn, l, rt, exact, narrow) / (2 * nz);
-				up = cur;
-				cur = dn;
-				rp += w;
-				ep += w;
-			}
-		} else if (!MODEL || fr.error) {
-			double up = lap_sample(clean, gx, Y0 - HS + r0 - 1, w, h, mode);
-			double cur = lap_sample(clean, gx, Y0 - HS + r0, w, h, mode);
-#pragma unroll 5
-			for (int r = 0; r < ROWS_PER_WAVE; ++r) {
-				int gy = Y0 - HS + r0 + r;
-				double dn = lap_sample(clean, gx, gy + 1, w, h, mode);
-				double l = lap_sample(clean, gx - 1, gy, w, h, mode);
-				double rt = lap_sample(clean, gx + 1, gy, w, h, mode);
-				double nz = fr.error[map_index(gx, gy, w, h, mode)];
-				bool in = gx >= 0 && gx < w && gy >= 0 && gy < h;
-				bad |= (in || wrap) && !(nz > 0) && !isnan(nz);
-				sh.s[r0 + r][lane] = laplacian(cur, up, dn, l, rt, exact, narrow) / (2 * nz);
-				up = cur;
-				cur = dn;
-			}
+		int tx, ty0, K;
+		if (iter == 0) {
+			int lr = item - fr.run0;
+			tx = lr % fr.tiles_x;
+			ty0 = lr / fr.tiles_x * RUN_TILES;
+			K = min(RUN_TILES, fr.tiles_y - ty0);
 		} else {
-			/* noise model: N from strip-shared 5x5 medians, four rows at a time */
-#pragma unroll 1
-			for (int g = 0; g < ROWS_PER_WAVE; g += 4) {
-				int gy0 = Y0 - HS + r0 + g;
-				double rows[8 * 5], med[4];
-#pragma unroll
-				for (int j = 0; j < 8; ++j)
-#pragma unroll
-					for (int c = 0; c < 5; ++c)
-						rows[j * 5 + c] = med_sample(clean, gx - 2 + c, gy0 - 2 + j, w, h, mode);
-				strip_median5x4(rows, med);
-#pragma unroll
-				for (int o = 0; o < 4; ++o) {
-					if (g + o >= ROWS_PER_WAVE)
-						break;
-					int gy = gy0 + o;
-					double cur = lap_sample(clean, gx, gy, w, h, mode);
-					double up = lap_sample(clean, gx, gy - 1, w, h, mode);
-					double dn = lap_sample(clean, gx, gy + 1, w, h, mode);
-					double l = lap_sample(clean, gx - 1, gy, w, h, mode);
-					double rt = lap_sample(clean, gx + 1, gy, w, h, mode);
-					int mi = map_index(gx, gy, w, h, mode);
-					double gg = fr.gain ? fr.gain[mi] : fr.g;
-					double rn = fr.readnoise ? fr.readnoise[mi] : fr.rn;
-					double nz = noise_model(med[o], gg, rn);
-					bool in = gx >= 0 && gx < w && gy >= 0 && gy < h;
-					bad |= (in || wrap) && !(nz > 0) && !isnan(nz);
-					sh.s[r0 + g + o][lane] = laplacian(cur, up, dn, l, rt, exact, narrow) / (2 * nz);
+			int lt = item - fr.tile0;
+			tx = lt % fr.tiles_x;
+			ty0 = lt / fr.tiles_x;
+			K = 1;
+			if (!threadIdx.x)
+				p.dirty[(size_t)(iter & 1) * p.ntiles + item] = 0;
+		}
+		const int w = fr.w, h = fr.h, X0 = tx * TILE_W, Y0 = ty0 * TILE_H, yb = Y0 - HS;
+		const int ncols = min(TILE_W, w - X0);
+		const unsigned long long cols = ((1ull << ncols) - 1) << HS;
+
+		for (int s = 0; s < K; ++s) {
+			/* per-step opaque thread index, as in significance_rows */
+			int tid = threadIdx.x;
+			asm volatile("" : "+v"(tid));
+			const int lane = tid & 63;
+			/* 1. S rows: the first tile needs rows [0, STEP_ROWS) of the run,
+			 * each later one TILE_H more */
+			if (s == 0) {
+				significance_rows<MODEL, EXACT, TAME, STEP_ROWS / 4>(sh, p, fr, X0, yb,
+					wave * (STEP_ROWS / 4), t_min, cols);
+			} else {
+				__syncthreads(); /* the last tile's medians are done with the ring */
+				significance_rows<MODEL, EXACT, TAME, TILE_H / 4>(sh, p, fr, X0, yb,
+					TILE_H * s + 2 * HS + wave * (TILE_H / 4), t_min, cols);
+			}
+			const int ty = Y0 + TILE_H * s; /* first frame row of the tile */
+			if (!tid) {
+				p.tile_pass[fr.tile0 + (ty0 + s) * fr.tiles_x + tx] = stamp;
+				work += (unsigned long long)(ncols * min(TILE_H, h - ty));
+				sh.na = 0;
+			}
+			__syncthreads();
+
+			/* 2. out-of-frame slots the medians read: the mapped in-frame slot
+			 * (median border mode), or 0 for CONSTANT */
+			const int r_lo = TILE_H * s, y_lo = yb + r_lo;
+			const bool edge = X0 - HS < 0 || X0 + TILE_W + HS > w || y_lo < 0 || y_lo + STEP_ROWS > h;
+			if (edge && !wrap) {
+				for (int k = tid; k < SW * STEP_ROWS; k += BLOCK) {
+					int sx = k & (SW - 1), sr = k / SW, x = X0 - HS + sx, y = y_lo + sr;
+					if ((x >= 0 && x < w && y >= 0 && y < h) || x < -2 || x > w + 1 || y < -2 || y > h + 1)
+						continue;
+					double v = 0.0;
+					if (mode != CUSMIC_CONSTANT)
+						v = sh.s[(border_index(y, h, mode) - yb) % RING][border_index(x, w, mode) - (X0 - HS)];
+					sh.s[(r_lo + sr) % RING][sx] = v;
 				}
+				__syncthreads();
 			}
-		}
-		if (bad)
-			sh.unsafe = 1;
-		__syncthreads();
 
-		/* 2. out-of-frame slots read by the 5x5 median: the mapped in-frame
-		 * slot (median border mode), or 0 for CONSTANT */
-		const bool edge = X0 - HS < 0 || Y0 - HS < 0 || X0 + TILE_W + HS > w || Y0 + TILE_H + HS > h;
-		if (edge && !wrap) {
-			for (int k = tid; k < SW * SH; k += BLOCK) {
-				int sx = k % SW, sy = k / SW, x = X0 - HS + sx, y = Y0 - HS + sy;
-				if ((x >= 0 && x < w && y >= 0 && y < h) || x < -2 || x > w + 1 || y < -2 || y > h + 1)
-					continue;
-				double v = 0.0;
-				if (mode != CUSMIC_CONSTANT)
-					v = sh.s[border_index(y, h, mode) - (Y0 - HS)][border_index(x, w, mode) - (X0 - HS)];
-				sh.s[sy][sx] = v;
+			/* 3. screened pixels of the tile: S' <= S, so only S > min(t, t_n)
+			 * can pass a threshold; with an invalid noise value among the rows
+			 * the medians read, every pixel is kept */
+			const bool unsafe = __ballot(lane < STEP_ROWS && sh.bad[(r_lo + lane) % RING]) != 0;
+			for (int j = 0; j < TILE_H / 4; ++j) {
+				int row = wave * (TILE_H / 4) + j;
+				if (ty + row >= h)
+					break;
+				unsigned long long m = unsafe ? cols : sh.screen[(r_lo + HS + row) % RING];
+				lds_append(((m >> lane) & 1) != 0, (uint16_t)((row << 6) | lane), sh.alist, &sh.na);
 			}
 			__syncthreads();
-		}
 
-		/* 3. screen: S' = S - med5(S) <= S, so only S > min(t, t_n) can pass a
-		 * threshold; a tile with an invalid noise value (<= 0) keeps every
-		 * pixel */
-		const bool unsafe = sh.unsafe != 0;
-		for (int k = tid; k < PW * PH; k += BLOCK) {
-			int sx = k % PW, sy = k / PW, x = X0 - HP + sx, y = Y0 - HP + sy;
-			bool a = x >= 0 && x < w && y >= 0 && y < h &&
-				 (unsafe || sh.s[sy + (HS - HP)][sx + (HS - HP)] > t_min);
-			lds_append(a, (uint16_t)k, sh.alist, &sh.na);
-		}
-		__syncthreads();
-
-		/* 4. S' at the screened pixels, stamped into the sparse plane; those
-		 * above t are buffered as seeds. Holes (S' = 0) and excluded pixels are
-		 * dropped by the consumers, which read the flags anyway. */
-		const int na = sh.na;
-		for (int base = 0; base < na; base += BLOCK) {
-			/* a batch adds at most BLOCK seeds */
-			__syncthreads();
-			if (sh.nseed + BLOCK > SEED_BUF)
-				flush_seeds(sh, p.list_b, seeds);
-			int idx = base + tid;
-			bool seed = false;
-			int i = 0;
-			if (idx < na) {
-				int pos = sh.alist[idx], sx = pos % PW, sy = pos / PW;
-				int x = X0 - HP + sx, y = Y0 - HP + sy;
-				i = y * w + x;
-				int cx = sx + (HS - HP), cy = sy + (HS - HP);
-				double v[25], m;
+			/* 4. S' at the screened pixels, stamped into the sparse plane; those
+			 * above t are buffered as seeds. Holes (S' = 0) and excluded pixels
+			 * are dropped by the consumers, which read the flags anyway. */
+			const int na = sh.na;
+			for (int base = 0; base < na; base += BLOCK) {
+				/* a batch adds at most BLOCK seeds */
+				__syncthreads();
+				if (sh.nseed + BLOCK > SEED_BUF)
+					flush_seeds(sh, p.list_b, seeds);
+				int idx = base + tid;
+				bool seed = false;
+				int i = 0;
+				if (idx < na) {
+					int pos = sh.alist[idx], row = pos >> 6, c = pos & 63;
+					i = (ty + row) * w + X0 - HS + c;
+					int rc = r_lo + HS + row;
+					double v[25], m;
 #pragma unroll
-				for (int dy = 0; dy < 5; ++dy)
+					for (int dy = 0; dy < 5; ++dy) {
+						int sl = (rc - 2 + dy) % RING;
 #pragma unroll
-					for (int dx = 0; dx < 5; ++dx)
-						v[dy * 5 + dx] = sh.s[cy - 2 + dy][cx - 2 + dx];
-				median25(v, &m);
-				double sp = sh.s[cy][cx] - m;
-				p.sprime[fr.off + i] = sp;
-				reinterpret_cast<uint16_t *>(p.flags + fr.off + i)[1] = (uint16_t)stamp;
-				seed = sp > t_cr;
+						for (int dx = 0; dx < 5; ++dx)
+							v[dy * 5 + dx] = sh.s[sl][c - 2 + dx];
+					}
+					median25(v, &m);
+					double sp = v[12] - m;
+					p.sprime[fr.off + i] = sp;
+					reinterpret_cast<uint16_t *>(p.flags + fr.off + i)[1] = (uint16_t)stamp;
+					seed = sp > t_cr;
+				}
+				lds_append(seed, list_entry(f, i), sh.seeds, &sh.nseed);
 			}
-			lds_append(seed, list_entry(f, i), sh.seeds, &sh.nseed);
 		}
 	}
 	flush_seeds(sh, p.list_b, seeds);
+	/* one atomic per workgroup: a counter that every tile hit would
+	 * serialise the whole pass in L2 */
+	if (!threadIdx.x && work)
+		atomicAdd(&p.work[iter], work);
+}
+
+/* Every frame has an error map. The pass is bound by load latency, so the
+ * register budget is capped at four waves per SIMD (128 VGPRs, no spills),
+ * one workgroup more per CU than the compiler would choose. */
+template <bool EXACT>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
+significance_maps(Params p, int iter)
+{
+	__shared__ SigShared sh;
+	if (p.status[1])
+		significance_pass<false, EXACT, false>(sh, p, iter);
+	else
+		significance_pass<false, EXACT, true>(sh, p, iter);
+}
+
+/* Some frame uses the noise model (5x5 medians of C in the dense part):
+ * two waves per SIMD. */
+template <bool EXACT>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2))) void
+significance_model(Params p, int iter)
+{
+	__shared__ SigShared sh;
+	if (p.status[1])
+		significance_pass<true, EXACT, false>(sh, p, iter);
+	else
+		significance_pass<true, EXACT, true>(sh, p, iter);
 }
 
 __device__ __forceinline__ double
@@ -1115,13 +1245,16 @@ fine(Params p, int iter)
 	}
 }
 
-/* S' at a pixel if this pass computed it there (0 at holes,
- * src/filters.cuh:137-146), else -inf: neither can pass a threshold t >= 0. */
+/* S' at pixel (x, y) of a frame if the last pass that processed its tile
+ * computed it there (0 at holes, src/filters.cuh:137-146), else -inf: neither
+ * can pass a threshold t >= 0. */
 __device__ __forceinline__ double
-sprime_at(const Params &p, long long g, unsigned stamp)
+sprime_at(const Params &p, long long off, int tile0, int tiles_x, int x, int y, int w)
 {
+	long long g = off + (long long)y * w + x;
 	uint32_t fw = p.flags[g];
-	return (fw >> STAMP_SHIFT) == stamp && !(fw & HOLE) ? p.sprime[g] : -INFINITY;
+	unsigned last = p.tile_pass[tile0 + (y / TILE_H) * tiles_x + x / TILE_W];
+	return (fw >> STAMP_SHIFT) == last && !(fw & HOLE) ? p.sprime[g] : -INFINITY;
 }
 
 /* First growth (src/filters.cuh:177-193 at t): every pixel next to a
@@ -1131,13 +1264,12 @@ grow_first(Params p, int iter)
 {
 	__shared__ BlockList<BLOCK * 9> out;
 	const unsigned n = p.nlist[nl_cand(p, iter)];
-	const unsigned stamp = pass_stamp(iter);
 	unsigned int *count = &p.nlist[nl_grow(p, iter)];
 	if (!threadIdx.x)
 		out.n = 0;
 	for (unsigned base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
 		unsigned e = base + threadIdx.x;
-		int f = 0, x = 0, y = 0, w = 1, h = 1;
+		int f = 0, x = 0, y = 0, w = 1, h = 1, tile0 = 0, tiles_x = 1;
 		long long off = 0;
 		if (e < n) {
 			unsigned long long entry = p.list_c[e];
@@ -1147,6 +1279,8 @@ grow_first(Params p, int iter)
 			w = fr.w;
 			h = fr.h;
 			off = fr.off;
+			tile0 = fr.tile0;
+			tiles_x = fr.tiles_x;
 			x = i % w;
 			y = i / w;
 		}
@@ -1155,7 +1289,8 @@ grow_first(Params p, int iter)
 			int qx = x + k % 3 - 1, qy = y + k / 3 - 1;
 			bool join = false;
 			int q = qy * w + qx;
-			if (e < n && qx >= 0 && qx < w && qy >= 0 && qy < h && sprime_at(p, off + q, stamp) > p.threshold)
+			if (e < n && qx >= 0 && qx < w && qy >= 0 && qy < h &&
+			    sprime_at(p, off, tile0, tiles_x, qx, qy, w) > p.threshold)
 				join = !(atomicOr(&p.flags[off + q], GROW) & GROW);
 			lds_append(join, list_entry(f, q), out.item, &out.n);
 		}
@@ -1171,12 +1306,11 @@ grow_second(Params p, int iter)
 {
 	__shared__ BlockList<BLOCK * 9> out;
 	const unsigned n = p.nlist[nl_grow(p, iter)];
-	const unsigned stamp = pass_stamp(iter);
 	if (!threadIdx.x)
 		out.n = 0;
 	for (unsigned base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
 		unsigned e = base + threadIdx.x;
-		int f = 0, x = 0, y = 0, w = 1, h = 1, fresh_n = 0, lost_n = 0;
+		int f = 0, x = 0, y = 0, w = 1, h = 1, tile0 = 0, tiles_x = 1, fresh_n = 0, lost_n = 0;
 		long long off = 0;
 		if (e < n) {
 			unsigned long long entry = p.list_b[e];
@@ -1186,6 +1320,8 @@ grow_second(Params p, int iter)
 			w = fr.w;
 			h = fr.h;
 			off = fr.off;
+			tile0 = fr.tile0;
+			tiles_x = fr.tiles_x;
 			x = i % w;
 			y = i / w;
 			atomicAnd(&p.flags[off + i], ~GROW);
@@ -1196,7 +1332,7 @@ grow_second(Params p, int iter)
 			bool fresh = false;
 			int q = qy * w + qx;
 			if (e < n && qx >= 0 && qx < w && qy >= 0 && qy < h &&
-			    sprime_at(p, off + q, stamp) > p.neighbor) {
+			    sprime_at(p, off, tile0, tiles_x, qx, qy, w) > p.neighbor) {
 				uint32_t old = atomicOr(&p.flags[off + q], CRAY);
 				if (!(old & CRAY)) {
 					fresh = true;
