@@ -9,8 +9,10 @@ resident in HBM. A step is one such call. Also reported:
   * per_frame: the same 64 frames as 64 separate plain-path calls;
   * fwd_bwd: 8 x 1024^2 noise-model frames, forward with donor recording then
     the adjoint of loss = sum(w * cleaned), vs forward alone;
-  * roofline: the detection kernel's algorithmic bytes (8 B of clean image +
-    8 B of error map per processed pixel) over its event-timed duration;
+  * roofline: the dense significance kernel's algorithmic bytes (8 B of clean
+    image + 8 B of error map per processed pixel) over its event-timed
+    duration; call_roofline: the whole call's algorithmic bytes
+    (17 + M + I_d (8 + E) + I_r 8 per pixel) over the step time;
   * cpu_baseline: the oracle (the C restatement, one thread) on a bounded
     sample of the same workload.
 Prints one JSON line on stdout; progress goes to stderr.
@@ -28,6 +30,21 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "cleaned Mpix/s fp64 maxiter=4: ragged-batch vs per-frame; fwd+bwd Mpix/s"
 HBM_PEAK_GBS = 8000.0
+SIG_BYTES_PER_PX = 16.0
+TRAFFIC_FILE = ROOT / "profiles" / "significance_maps_ragged64_traffic.json"
+
+
+def significance_traffic(launches):
+    """HBM bytes per significance launch from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE of the same workload, scaled by a copy kernel of
+    known size), or None when there is none."""
+    try:
+        rec = json.loads(TRAFFIC_FILE.read_text())
+    except (OSError, ValueError):
+        return None, None
+    if rec.get("launches") != launches:
+        return None, None
+    return round(rec["hbm_bytes_per_launch"]), TRAFFIC_FILE.name
 
 
 def log(*a):
@@ -59,6 +76,9 @@ def main():
     ap.add_argument("--no-per-frame", action="store_true")
     ap.add_argument("--no-backward", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibration-copy", action="store_true",
+                    help="start with a 1 GiB device-to-device copy, whose hardware counter "
+                         "readings scale FETCH_SIZE/WRITE_SIZE to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every GPU box in this setup has one GPU, and the "
@@ -73,6 +93,11 @@ def main():
 
     dev = torch.device("cuda", 0)
     _lib.lib()
+    if args.calibration_copy:
+        a = torch.ones(1 << 27, dtype=torch.float64, device=dev)
+        torch.empty_like(a).copy_(a)
+        torch.cuda.synchronize()
+        del a
     log("generating the ragged batch on the GPU")
     batch = scenes.config3(dev, n=args.frames)
     fr = engine.Frames()
@@ -103,20 +128,20 @@ def main():
     value = mpix / (ms_step / 1e3)
     log(f"ragged: {ms_step:.3f} ms/step, {value:.1f} Mpix/s")
     flagged = int(mask.sum())
-    passes = step()[3]
 
-    detect_ms = float(np.mean([p["detect_ms"] for p in profiles]))
-    detect_px = float(np.mean([p["detect_pixels"] for p in profiles]))
-    launches = profiles[0]["detect_launches"]
-    bytes_per_px = 16.0  # clean image 8 B + error map 8 B per processed pixel
-    achieved = detect_px * bytes_per_px / (detect_ms / 1e3) / 1e9
-    per_launch_bytes = detect_px * bytes_per_px / launches
-    stage_ms = {k: float(np.mean([p[k] for p in profiles]))
-                for k in ("detect_ms", "replace_ms", "prologue_ms", "epilogue_ms")}
-    # algorithmic bytes of the whole call (B = 17 + M + I_d (8 + E) + I_r 8)
-    mean_passes = float(np.mean(passes))
-    call_bytes = total * (17 + 8 * 0 + mean_passes * 16 + max(0.0, mean_passes - 1) * 8) \
-        + 0.0
+    def mean(key):
+        return float(np.mean([p[key] for p in profiles]))
+
+    # dominant kernel: the dense significance pass (16 B read per processed
+    # pixel: clean image and error map), timed with hipEvents on its stream
+    sig_ms, sig_px, launches = mean("significance_ms"), mean("detect_pixels"), profiles[0]["detect_launches"]
+    sig_bytes = SIG_BYTES_PER_PX * sig_px
+    achieved = sig_bytes / (sig_ms / 1e3) / 1e9
+    traffic, traffic_note = significance_traffic(launches)
+    # whole call, algorithmic bytes B = 17 + M + I_d (8 + E) + I_r 8 per frame
+    # pixel with M = 8 (error map) and E = 8 (its re-read per pass)
+    call_bytes = 25.0 * total + 16.0 * mean("detect_pass_pixels") + 8.0 * mean("replace_pass_pixels")
+    call_gbs = call_bytes / (ms_step / 1e3) / 1e9
     result = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mpix/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
@@ -126,14 +151,21 @@ def main():
                                "per-frame error maps, 2e-4 CR tracks/px, contrast 3, thresholds 5/3, "
                                "maxiter 4, mirror, one remove_cosmics_ragged_device call",
                    "frames": len(batch), "megapixels": round(mpix, 3), "maxiter": 4,
-                   "flagged_pixels": flagged, "detection_passes_mean": mean_passes},
+                   "flagged_pixels": flagged,
+                   "detection_passes_per_pixel": round(mean("detect_pass_pixels") / total, 4),
+                   "replacement_passes_per_pixel": round(mean("replace_pass_pixels") / total, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "detect", "algorithmic_bytes_per_launch": round(per_launch_bytes),
-                     "avg_launch_ms": round(detect_ms / launches, 4)},
-        "stages_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "kernel": "significance_maps",
+                     "algorithmic_bytes_per_launch": round(sig_bytes / launches),
+                     "avg_launch_ms": round(sig_ms / launches, 4), "launches": launches,
+                     "traffic_source": traffic_note},
+        "call_roofline": {"bytes_per_pixel": round(call_bytes / total, 3),
+                          "achieved": round(call_gbs, 1), "unit": "GB/s",
+                          "frac": round(call_gbs / HBM_PEAK_GBS, 4)},
+        "stages_ms": {k: round(mean(k), 4) for k in
+                      ("prologue_ms", "significance_ms", "detect_ms", "replace_ms", "epilogue_ms")},
         "lists_per_call": {k: int(profiles[0][k]) for k in ("seeds", "candidates", "grown", "widened")},
-        "call_roofline_frac": round(call_bytes / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
     }
 
     if not args.no_per_frame:

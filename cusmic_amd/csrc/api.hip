@@ -336,6 +336,7 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	for (int it = 0; it < maxiter; ++it) {
 		unsigned grid = unsigned(it == 0 ? std::min(nruns, sig_resident) : sig_resident);
 		hipLaunchKernelGGL(sig, dim3(grid), dim3(BLOCK), 0, stream, p, it);
+		tm.mark(CUSMIC_PROFILE_SIGNIFICANCE_MS);
 		hipLaunchKernelGGL(fine, dim3(list_grid), dim3(BLOCK), 0, stream, p, it);
 		hipLaunchKernelGGL(grow_first, dim3(list_grid), dim3(BLOCK), 0, stream, p, it);
 		hipLaunchKernelGGL(grow_second, dim3(list_grid), dim3(BLOCK), 0, stream, p, it);
@@ -364,7 +365,19 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 		for (int k = 0; k < CUSMIC_PROFILE_SIZE; ++k)
 			profile[k] = 0;
 		tm.collect(profile);
+		profile[CUSMIC_PROFILE_DETECT_MS] += profile[CUSMIC_PROFILE_SIGNIFICANCE_MS];
 		profile[CUSMIC_PROFILE_DETECT_LAUNCHES] = maxiter;
+		/* frame f ran detection passes until one found nothing new (or
+		 * maxiter), and a replacement pass after each that found pixels */
+		for (size_t f = 0; f < nf; ++f) {
+			double px = double(frames[f].w) * frames[f].h;
+			for (int it = 0; it < maxiter; ++it) {
+				profile[CUSMIC_PROFILE_DETECT_PASS_PIXELS] += px;
+				if (!host[size_t(it) * nf + f])
+					break;
+				profile[CUSMIC_PROFILE_REPLACE_PASS_PIXELS] += px;
+			}
+		}
 		for (auto v : work)
 			profile[CUSMIC_PROFILE_DETECT_PIXELS] += double(v);
 		profile[CUSMIC_PROFILE_FLAGGED] = nl[0];
