@@ -148,11 +148,12 @@ device_cus()
 /* Frame table of a batch: offsets and tile numbering. */
 std::vector<Frame>
 frame_table(const cusmic_image *ims, size_t nf, const cusmic_options &o, const double *fg,
-	const double *frn, int *ntiles_out, int *nchunks_out = nullptr, int *nruns_out = nullptr)
+	const double *frn, int *ntiles_out, int *nchunks_out = nullptr, int *nruns_out = nullptr,
+	int *nblocks_out = nullptr)
 {
 	std::vector<Frame> frames(nf);
 	long long off = 0;
-	int tiles = 0, chunks = 0, runs = 0;
+	int tiles = 0, chunks = 0, runs = 0, blocks = 0;
 	for (size_t f = 0; f < nf; ++f) {
 		const cusmic_image &im = ims[f];
 		Frame &fr = frames[f];
@@ -172,10 +173,14 @@ frame_table(const cusmic_image *ims, size_t nf, const cusmic_options &o, const d
 		fr.tile0 = tiles;
 		fr.chunk0 = chunks;
 		fr.run0 = runs;
+		fr.block0 = blocks;
+		fr.blocks_x = (fr.w + BLK - 1) / BLK;
 		long long px = (long long)fr.w * fr.h;
+		long long nb = (long long)fr.blocks_x * ((fr.h + BLK - 1) / BLK);
 		if ((long long)tiles + (long long)fr.tiles_x * fr.tiles_y > INT_MAX ||
-			(long long)chunks + (px + CHUNK - 1) / CHUNK > INT_MAX)
+			(long long)chunks + (px + CHUNK - 1) / CHUNK > INT_MAX || blocks + nb > INT_MAX)
 			throw std::invalid_argument("batch is too large");
+		blocks += int(nb);
 		tiles += fr.tiles_x * fr.tiles_y;
 		chunks += int((px + CHUNK - 1) / CHUNK);
 		runs += fr.tiles_x * runs_y(fr.tiles_y);
@@ -186,6 +191,8 @@ frame_table(const cusmic_image *ims, size_t nf, const cusmic_options &o, const d
 		*nchunks_out = chunks;
 	if (nruns_out)
 		*nruns_out = runs;
+	if (nblocks_out)
+		*nblocks_out = blocks;
 	return frames;
 }
 
@@ -232,8 +239,8 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	const double *frn, int laplacian, double *clean, uint8_t *mask, int32_t *donors,
 	int32_t *passes, hipStream_t stream, double *profile = nullptr)
 {
-	int ntiles = 0, nchunks = 0, nruns = 0;
-	std::vector<Frame> frames = frame_table(ims, nf, o, fg, frn, &ntiles, &nchunks, &nruns);
+	int ntiles = 0, nchunks = 0, nruns = 0, nblocks = 0;
+	std::vector<Frame> frames = frame_table(ims, nf, o, fg, frn, &ntiles, &nchunks, &nruns, &nblocks);
 	size_t total = 0;
 	for (const auto &fr : frames)
 		total += size_t(fr.w) * fr.h;
@@ -243,18 +250,16 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	size_t n_counts = nf * (sizeof(Counts) / 4);
 	size_t n_todo = size_t(std::max(maxiter, 1)) * nf;
 	size_t n_queue = size_t(std::max(maxiter, 1));
-	size_t n_nlist = 3 + 5 * size_t(std::max(maxiter, 1));
-	size_t n_dirty = 2 * size_t(ntiles);
-	size_t words = n_counts + n_todo + n_queue + n_nlist + 2 + n_dirty;
+	size_t n_nlist = 3 + 6 * size_t(std::max(maxiter, 1));
+	size_t words = n_counts + n_todo + n_queue + n_nlist + 2 + 3 * size_t(nblocks);
 	size_t off_frames = 0;
 	size_t off_words = align_up(nf * sizeof(Frame), 256);
 	size_t off_dlist = align_up(off_words + words * 4, 256);
-	size_t off_work = align_up(off_dlist + n_dirty * 4, 256);
-	size_t off_tframe = align_up(off_work + n_queue * 8, 256);
-	size_t off_cframe = align_up(off_tframe + size_t(ntiles) * 4, 256);
+	size_t off_clist = align_up(off_dlist + size_t(nblocks) * 8, 256);
+	size_t off_work = align_up(off_clist + size_t(nblocks) * 8, 256);
+	size_t off_cframe = align_up(off_work + n_queue * 8, 256);
 	size_t off_rframe = align_up(off_cframe + size_t(nchunks) * 4, 256);
-	size_t off_tpass = align_up(off_rframe + size_t(nruns) * 4, 256);
-	size_t off_flags = align_up(off_tpass + size_t(ntiles) * 4, 256);
+	size_t off_flags = align_up(off_rframe + size_t(nruns) * 4, 256);
 	size_t planes = maxiter ? total : 0;
 	size_t off_sprime = align_up(off_flags + planes * 4, 256);
 	size_t off_list = align_up(off_sprime + planes * 8, 256);
@@ -286,14 +291,16 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	p.queue = w32 + n_counts + n_todo;
 	p.nlist = p.queue + n_queue;
 	p.status = reinterpret_cast<int *>(p.nlist + n_nlist);
-	p.dirty = reinterpret_cast<unsigned int *>(p.status + 2);
-	p.dirty_list = reinterpret_cast<unsigned int *>(base + off_dlist);
+	p.block_pass = reinterpret_cast<unsigned int *>(p.status + 2);
+	p.changed_tag = p.block_pass + nblocks;
+	p.dirty_tag = p.changed_tag + nblocks;
+	p.changed_list = reinterpret_cast<unsigned long long *>(base + off_clist);
+	p.nblocks = nblocks;
+	p.dirty_list = reinterpret_cast<unsigned long long *>(base + off_dlist);
 	p.work = reinterpret_cast<unsigned long long *>(base + off_work);
-	p.tile_frame = reinterpret_cast<int *>(base + off_tframe);
 	p.chunk_frame = reinterpret_cast<int *>(base + off_cframe);
 	p.run_frame = reinterpret_cast<int *>(base + off_rframe);
 	p.nruns = nruns;
-	p.tile_pass = reinterpret_cast<unsigned int *>(base + off_tpass);
 	p.nchunks = nchunks;
 	p.flags = reinterpret_cast<uint32_t *>(base + off_flags);
 	p.sprime = reinterpret_cast<double *>(base + off_sprime);
@@ -301,7 +308,7 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	p.list_b = reinterpret_cast<unsigned long long *>(base + off_list_b);
 	p.list_c = reinterpret_cast<unsigned long long *>(base + off_list_c);
 
-	unsigned table_grid = unsigned(std::min<long long>(((long long)ntiles + nchunks + nruns + 255) / 256, 4096));
+	unsigned table_grid = unsigned(std::min<long long>(((long long)nchunks + nruns + 255) / 256, 4096));
 	hipLaunchKernelGGL(set_tables, dim3(table_grid), dim3(256), 0, stream, p);
 	if (!maxiter) {
 		hipLaunchKernelGGL(copy_without_detection, dim3(nchunks), dim3(BLOCK), 0, stream, p);
@@ -333,9 +340,19 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&sig_per_cu, sig, BLOCK, 0));
 	const int sig_resident = cus * std::max(sig_per_cu, 1);
 	unsigned list_grid = unsigned(cus) * 4;
+	auto blocks = model ? (laplacian ? significance_blocks<true, false> : significance_blocks<true, true>)
+			    : (laplacian ? significance_blocks<false, false> : significance_blocks<false, true>);
+	int blocks_per_cu = 0;
+	hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, blocks, BLOCK, 0));
 	for (int it = 0; it < maxiter; ++it) {
-		unsigned grid = unsigned(it == 0 ? std::min(nruns, sig_resident) : sig_resident);
-		hipLaunchKernelGGL(sig, dim3(grid), dim3(BLOCK), 0, stream, p, it);
+		if (it == 0)
+			hipLaunchKernelGGL(sig, dim3(unsigned(std::min(nruns, sig_resident))), dim3(BLOCK), 0,
+				stream, p, it);
+		else {
+			hipLaunchKernelGGL(expand_dirty, dim3(unsigned(cus) * 2), dim3(BLOCK), 0, stream, p, it);
+			hipLaunchKernelGGL(blocks, dim3(unsigned(cus * std::max(blocks_per_cu, 1))), dim3(BLOCK),
+				0, stream, p, it);
+		}
 		tm.mark(CUSMIC_PROFILE_SIGNIFICANCE_MS);
 		hipLaunchKernelGGL(fine, dim3(list_grid), dim3(BLOCK), 0, stream, p, it);
 		hipLaunchKernelGGL(grow_first, dim3(list_grid), dim3(BLOCK), 0, stream, p, it);

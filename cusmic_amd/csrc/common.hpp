@@ -25,10 +25,15 @@ constexpr int BLOCK = 256;
  * tiles of a run are computed once. */
 constexpr int RUN_TILES = 8;
 
+/* Passes after the first recompute detection on BLK x BLK blocks marked
+ * dirty by the previous replacement. */
+constexpr int BLK = 8;
+
 /* Per-pixel state word (u32) of the flags plane: bits 0-7 are flags, bits
  * 16-31 hold the stamp of the last pass that computed S' at the pixel. S' at a
  * pixel is current iff that stamp equals the stamp of the last pass that
- * processed the pixel's tile (Params::tile_pass). */
+ * processed the pixel: the first pass processes every pixel, later passes the
+ * dirty blocks (Params::block_pass). */
 constexpr uint32_t HOLE = 1;    /* input pixel is nonfinite */
 constexpr uint32_t EXCL = 2;    /* hole or masked: never a seed or donor */
 constexpr uint32_t CRAY = 4;    /* flagged as cosmic ray (the output mask) */
@@ -53,6 +58,7 @@ struct Frame {
 	int tile0, tiles_x, tiles_y;
 	int chunk0;            /* first CHUNK-pixel chunk of the frame */
 	int run0;              /* first first-pass run of the frame */
+	int block0, blocks_x;  /* first dirty block, blocks per block row */
 };
 
 /* Pointwise passes stream frames in chunks of CHUNK consecutive pixels. */
@@ -79,21 +85,28 @@ struct Params {
 	int *todo;                       /* [maxiter][nf] new pixels per pass */
 	unsigned int *queue;             /* [maxiter] tile queue heads */
 	unsigned int *nlist;             /* [0] list length, [1] hole count,
-					  * [2 + iter] dirty tiles, [2 + M + iter] seeds,
+					  * [2 + iter] dirty blocks, [2 + M + iter] seeds,
 					  * [2 + 2M + iter] candidates, [2 + 3M + iter] grown,
-					  * [2 + 4M + iter] widened replacements (iter = M: fill) */
-	unsigned int *dirty;             /* [2][ntiles] dirty flags */
-	unsigned int *dirty_list;        /* [2][ntiles] dirty tile lists */
+					  * [2 + 4M + iter] widened replacements (iter = M: fill),
+					  * [3 + 5M + iter] changed blocks */
+	unsigned int *changed_tag;       /* [nblocks] stamp of the last pass whose
+					  * preceding replacement changed a pixel of
+					  * the block */
+	unsigned int *dirty_tag;         /* [nblocks] stamp of the last pass that
+					  * listed the block as dirty */
+	unsigned long long *changed_list; /* frame << 32 | block, per pass */
+	unsigned long long *dirty_list;  /* frame << 32 | block of the blocks a pass
+					  * after the first processes */
 	int *status;                     /* [0] error bits, [1] nonzero if a clean
 					  * sample is infinite (D + bg overflowed) */
 	unsigned long long *work;        /* [maxiter] in-frame pixels of processed tiles */
-	int *tile_frame;                 /* [ntiles] frame of each tile */
 	int *chunk_frame;                /* [nchunks] frame of each chunk */
 	int nchunks;
 	int *run_frame;                  /* [nruns] frame of each first-pass run */
 	int nruns;
-	unsigned int *tile_pass;         /* [ntiles] stamp of the last pass that
-					  * processed the tile */
+	int nblocks;
+	unsigned int *block_pass;        /* [nblocks] stamp of the last pass after
+					  * the first that processed the block, or 0 */
 };
 
 /* Tiles per run, in run order: frame-major, then row-major over blocks of
@@ -106,12 +119,20 @@ runs_y(int tiles_y)
 
 constexpr int ERR_NO_DONORS = 1;
 
+/* Stamp of pass iter (0 is never a stamp): S' stamps, block tags. */
+__device__ __forceinline__ unsigned
+pass_stamp(int iter)
+{
+	return unsigned(iter % 65535) + 1;
+}
+
 /* Offsets of the per-pass list counters in Params::nlist. */
 __device__ __forceinline__ int nl_dirty(const Params &p, int it) { return 2 + it; }
 __device__ __forceinline__ int nl_seed(const Params &p, int it) { return 2 + p.maxiter + it; }
 __device__ __forceinline__ int nl_cand(const Params &p, int it) { return 2 + 2 * p.maxiter + it; }
 __device__ __forceinline__ int nl_grow(const Params &p, int it) { return 2 + 3 * p.maxiter + it; }
 __device__ __forceinline__ int nl_wide(const Params &p, int it) { return 2 + 4 * p.maxiter + it; }
+__device__ __forceinline__ int nl_changed(const Params &p, int it) { return 3 + 5 * p.maxiter + it; }
 
 /* scipy.ndimage border mapping of coordinate x on an axis of n samples;
  * -1 means a CONSTANT-mode zero (src/filters.cuh:24-44). */
@@ -169,6 +190,20 @@ find_frame(const Frame *frames, int nf, int tile)
 	while (lo < hi) {
 		int mid = (lo + hi + 1) >> 1;
 		if (frames[mid].tile0 <= tile)
+			lo = mid;
+		else
+			hi = mid - 1;
+	}
+	return lo;
+}
+
+__device__ __forceinline__ int
+find_frame_of_block(const Frame *frames, int nf, int block)
+{
+	int lo = 0, hi = nf - 1;
+	while (lo < hi) {
+		int mid = (lo + hi + 1) >> 1;
+		if (frames[mid].block0 <= block)
 			lo = mid;
 		else
 			hi = mid - 1;

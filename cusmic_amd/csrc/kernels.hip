@@ -52,8 +52,9 @@ lds_append(bool pred, T item, T *list, int *count)
 		list[base + __popcll(m & ((1ull << lane) - 1))] = item;
 }
 
+template <class T>
 __device__ __forceinline__ void
-list_push(bool pred, unsigned long long item, unsigned long long *list, unsigned int *count)
+list_push(bool pred, T item, T *list, unsigned int *count)
 {
 	unsigned long long m = __ballot(pred);
 	if (!m)
@@ -151,19 +152,14 @@ tile_of(const Frame &fr, int tile)
 	return t;
 }
 
-/* tile -> frame, chunk -> frame and run -> frame tables (binary search once
- * per entry) */
+/* chunk -> frame and run -> frame tables (binary search once per entry) */
 __global__ void
 set_tables(Params p)
 {
-	for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < p.ntiles + p.nchunks + p.nruns;
+	for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < p.nchunks + p.nruns;
 	     t += gridDim.x * blockDim.x) {
-		if (t < p.ntiles) {
-			p.tile_frame[t] = find_frame(p.frames, p.nf, t);
-			continue;
-		}
-		const bool chunk = t < p.ntiles + p.nchunks;
-		int c = chunk ? t - p.ntiles : t - p.ntiles - p.nchunks, lo = 0, hi = p.nf - 1;
+		const bool chunk = t < p.nchunks;
+		int c = chunk ? t : t - p.nchunks, lo = 0, hi = p.nf - 1;
 		while (lo < hi) {
 			int mid = (lo + hi + 1) >> 1;
 			if ((chunk ? p.frames[mid].chunk0 : p.frames[mid].run0) <= c)
@@ -376,52 +372,72 @@ rank_key(double v, int r, int cnt_neg0)
 	return order_key(v);
 }
 
-/* Mark the tiles whose detection may change after pixel (x, y) changed. */
-__device__ void
-mark_dirty(const Params &p, const Frame &fr, int x, int y, int next)
+/* Record that pixel i of frame f changed (changed = this lane's pixel did)
+ * in the replacement before pass next: the first such pixel of a block puts
+ * the block on that pass's changed list (one atomic per wave). */
+__device__ __forceinline__ void
+mark_dirty(const Params &p, bool changed, int f, int i, int next)
 {
-	int tx[3], ty[3], nx = 0, ny = 0;
-	if (p.mode == CUSMIC_WRAP) {
-		if (fr.w <= 2 * REACH + 1 || fr.tiles_x <= 3) {
-			nx = -1;
-		} else {
-			int xs[3] = {x - REACH, x, x + REACH};
-			for (int k = 0; k < 3; ++k) {
-				int v = xs[k] < 0 ? xs[k] + fr.w : xs[k] >= fr.w ? xs[k] - fr.w : xs[k];
-				tx[nx++] = v / TILE_W;
-			}
-		}
-		if (fr.h <= 2 * REACH + 1 || fr.tiles_y <= 3) {
-			ny = -1;
-		} else {
-			int ys[3] = {y - REACH, y, y + REACH};
-			for (int k = 0; k < 3; ++k) {
-				int v = ys[k] < 0 ? ys[k] + fr.h : ys[k] >= fr.h ? ys[k] - fr.h : ys[k];
-				ty[ny++] = v / TILE_H;
-			}
-		}
-	} else {
-		int a = max(0, x - REACH) / TILE_W, b = min(fr.w - 1, x + REACH) / TILE_W;
-		for (int v = a; v <= b; ++v)
-			tx[nx++] = v;
-		a = max(0, y - REACH) / TILE_H;
-		b = min(fr.h - 1, y + REACH) / TILE_H;
-		for (int v = a; v <= b; ++v)
-			ty[ny++] = v;
+	const unsigned tag = pass_stamp(next);
+	unsigned b = 0;
+	bool fresh = false;
+	if (changed) {
+		const Frame fr = p.frames[f];
+		const int x = i % fr.w, y = i / fr.w;
+		b = unsigned(fr.block0 + (y / BLK) * fr.blocks_x + x / BLK);
+		fresh = __atomic_load_n(&p.changed_tag[b], __ATOMIC_RELAXED) != tag &&
+			atomicExch(&p.changed_tag[b], tag) != tag;
 	}
-	int cx = nx < 0 ? fr.tiles_x : nx, cy = ny < 0 ? fr.tiles_y : ny;
-	unsigned int *flag = p.dirty + (size_t)(next & 1) * p.ntiles;
-	unsigned int *list = p.dirty_list + (size_t)(next & 1) * p.ntiles;
-	for (int j = 0; j < cy; ++j)
-		for (int i = 0; i < cx; ++i) {
-			int tile = fr.tile0 + (ny < 0 ? j : ty[j]) * fr.tiles_x + (nx < 0 ? i : tx[i]);
-			/* most tiles are already marked by a neighbour: test before the
-			 * returning atomic */
-			if (!__atomic_load_n(&flag[tile], __ATOMIC_RELAXED) && !atomicExch(&flag[tile], 1u)) {
-				unsigned slot = atomicAdd(&p.nlist[nl_dirty(p, next)], 1u);
-				list[slot] = tile;
-			}
+	list_push(fresh, list_entry(f, (int)b), p.changed_list, &p.nlist[nl_changed(p, next)]);
+}
+
+/* Blocks of pass iter: every block within one block (Chebyshev) of a changed
+ * block, since REACH < BLK; under WRAP two blocks, because a partial block at
+ * the seam is narrower than REACH. Each joins the dirty list once (tagged with
+ * the pass), through a per-workgroup LDS buffer and one atomic per flush. */
+__global__ __launch_bounds__(BLOCK) void
+expand_dirty(Params p, int iter)
+{
+	static_assert(REACH < BLK, "a change reaches the adjacent blocks only");
+	__shared__ BlockList<BLOCK * 9> out;
+	const unsigned n = p.nlist[nl_changed(p, iter)], tag = pass_stamp(iter);
+	const int r = p.mode == CUSMIC_WRAP ? 2 : 1, side = 2 * r + 1;
+	unsigned int *count = &p.nlist[nl_dirty(p, iter)];
+	if (!threadIdx.x)
+		out.n = 0;
+	for (unsigned base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
+		const unsigned e = base + threadIdx.x;
+		int f = 0, bx = 0, by = 0, nbx = 1, nby = 1, block0 = 0;
+		if (e < n) {
+			unsigned long long entry = p.changed_list[e];
+			f = (int)(entry >> 32);
+			const Frame fr = p.frames[f];
+			block0 = fr.block0;
+			nbx = fr.blocks_x;
+			nby = (fr.h + BLK - 1) / BLK;
+			int lb = (int)(entry & 0xffffffffu) - block0;
+			bx = lb % nbx;
+			by = lb / nbx;
 		}
+		/* rounds of at most nine neighbours, one buffer flush each */
+		for (int k0 = 0; k0 < side * side; k0 += 9) {
+			__syncthreads();
+			for (int k = k0; k < k0 + 9 && k < side * side; ++k) {
+				int qx = bx + k % side - r, qy = by + k / side - r;
+				bool fresh = false;
+				if (p.mode == CUSMIC_WRAP) {
+					qx = (qx % nbx + nbx) % nbx;
+					qy = (qy % nby + nby) % nby;
+				}
+				const int q = block0 + qy * nbx + qx;
+				if (e < n && qx >= 0 && qx < nbx && qy >= 0 && qy < nby)
+					fresh = __atomic_load_n(&p.dirty_tag[q], __ATOMIC_RELAXED) != tag &&
+						atomicExch(&p.dirty_tag[q], tag) != tag;
+				lds_append(fresh, list_entry(f, q), out.item, &out.n);
+			}
+			block_list_flush(out, p.dirty_list, count);
+		}
+	}
 }
 
 /* Value of a replacement from the two middle donors (src/replace.cuh:141-150):
@@ -639,20 +655,23 @@ replace_pass(Params p, int iter, int fill)
 	unsigned int *wide = &p.nlist[nl_wide(p, fill ? p.maxiter : iter)];
 	for (unsigned base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
 		unsigned e = base + threadIdx.x;
-		bool widen = false;
+		bool widen = false, changed = false;
 		unsigned long long entry = 0;
+		int f = 0, i = 0;
 		if (e < n) {
 			entry = p.list[e];
-			int f = (int)(entry >> 32), i = (int)(entry & 0xffffffffu);
+			f = (int)(entry >> 32);
+			i = (int)(entry & 0xffffffffu);
 			if (replace_allowed(p, f, iter, fill)) {
 				const Frame fr = p.frames[f];
 				int r = replace_near(p, fr, i, fill ? nullptr : p.donors);
 				widen = r < 0;
-				if (r > 0 && !fill && iter + 1 < p.maxiter)
-					mark_dirty(p, fr, i % fr.w, i / fr.w, iter + 1);
+				changed = r > 0;
 			}
 		}
 		list_push(widen, entry, p.list_c, wide);
+		if (!fill && iter + 1 < p.maxiter)
+			mark_dirty(p, changed, f, i, iter + 1);
 	}
 }
 
@@ -665,8 +684,8 @@ replace_widened(Params p, int iter, int fill)
 		int f = (int)(entry >> 32), i = (int)(entry & 0xffffffffu);
 		const Frame fr = p.frames[f];
 		bool changed = replace_wide(p, fr, i, fill ? nullptr : p.donors);
-		if (changed && !fill && iter + 1 < p.maxiter)
-			mark_dirty(p, fr, i % fr.w, i / fr.w, iter + 1);
+		if (!fill && iter + 1 < p.maxiter)
+			mark_dirty(p, changed, f, i, iter + 1);
 	}
 }
 
@@ -680,13 +699,14 @@ replace_widened(Params p, int iter, int fill)
  *                 sparse plane stamped with the pass; seeds (S' > t) go to a
  *                 list. The first pass walks runs of RUN_TILES tiles down a
  *                 tile column, so rows shared by two tiles are computed once;
- *                 later passes process the tiles marked dirty.
+ *                 later passes process the BLK x BLK blocks marked dirty, one
+ *                 wave per block.
  *   fine          per seed: F = max((m3 - med7(m3)) / N, 0.01), m3 = med3(C);
  *                 S' / F > contrast puts the seed on the candidate list
  *   grow_first    per candidate: neighbours with S' > t join the growth list
  *   grow_second   per grown pixel: neighbours with S' > t_n are flagged; a
  *                 pixel flagged for the first time is new (accumulate)
- * A tile that a pass skips had no change of the clean image within REACH of
+ * A block that a pass skips had no change of the clean image within REACH of
  * its pixels since the pass that last processed it, so its S' values and the
  * growth of its candidates are still those of that pass: the sparse passes
  * give the reference's dense result exactly.
@@ -699,12 +719,6 @@ constexpr int RING = 40;                   /* rows of S kept in LDS */
 static_assert(SW == 64, "the significance region is one wave wide");
 static_assert(STEP_ROWS % 4 == 0 && TILE_H % 4 == 0, "four waves share the rows of a step");
 static_assert(RING >= STEP_ROWS && RING <= 64, "the ring holds the rows of one tile");
-
-__device__ __forceinline__ unsigned
-pass_stamp(int iter)
-{
-	return unsigned(iter % 65535) + 1;
-}
 
 /* max(v, 0) as the reference writes it (v < 0 ? 0 : v), which keeps a NaN.
  * Without infinite samples (TAME) no NaN can arise and one max instruction
@@ -945,8 +959,7 @@ significance_pass(SigShared &sh, const Params &p, int iter)
 	const bool wrap = mode == CUSMIC_WRAP;
 	const double t_cr = p.threshold, t_min = fmin(p.threshold, p.neighbor);
 	const unsigned stamp = pass_stamp(iter);
-	const unsigned limit = iter == 0 ? (unsigned)p.nruns : p.nlist[nl_dirty(p, iter)];
-	const unsigned int *dlist = p.dirty_list + (size_t)(iter & 1) * p.ntiles;
+	const unsigned limit = (unsigned)p.nruns;
 	unsigned int *seeds = &p.nlist[nl_seed(p, iter)];
 	unsigned next = 0;
 	unsigned long long work = 0; /* in-frame pixels processed, thread 0 */
@@ -959,7 +972,7 @@ significance_pass(SigShared &sh, const Params &p, int iter)
 		__syncthreads();
 		if (!threadIdx.x) {
 			unsigned q = next;
-			sh.item = q < limit ? (iter == 0 ? (int)q : (int)dlist[q]) : -1;
+			sh.item = q < limit ? (int)q : -1;
 			if (q < limit)
 				next = atomicAdd(&p.queue[iter], 1u);
 		}
@@ -968,22 +981,10 @@ significance_pass(SigShared &sh, const Params &p, int iter)
 		if (item < 0)
 			break;
 		/* the run: tile column tx, tile rows [ty0, ty0 + K) of frame f */
-		const int f = iter == 0 ? p.run_frame[item] : p.tile_frame[item];
+		const int f = p.run_frame[item];
 		const Frame fr = p.frames[f];
-		int tx, ty0, K;
-		if (iter == 0) {
-			int lr = item - fr.run0;
-			tx = lr % fr.tiles_x;
-			ty0 = lr / fr.tiles_x * RUN_TILES;
-			K = min(RUN_TILES, fr.tiles_y - ty0);
-		} else {
-			int lt = item - fr.tile0;
-			tx = lt % fr.tiles_x;
-			ty0 = lt / fr.tiles_x;
-			K = 1;
-			if (!threadIdx.x)
-				p.dirty[(size_t)(iter & 1) * p.ntiles + item] = 0;
-		}
+		const int lr = item - fr.run0, tx = lr % fr.tiles_x, ty0 = lr / fr.tiles_x * RUN_TILES;
+		const int K = min(RUN_TILES, fr.tiles_y - ty0);
 		const int w = fr.w, h = fr.h, X0 = tx * TILE_W, Y0 = ty0 * TILE_H, yb = Y0 - HS;
 		const int ncols = min(TILE_W, w - X0);
 		const unsigned long long cols = ((1ull << ncols) - 1) << HS;
@@ -1005,7 +1006,6 @@ significance_pass(SigShared &sh, const Params &p, int iter)
 			}
 			const int ty = Y0 + TILE_H * s; /* first frame row of the tile */
 			if (!tid) {
-				p.tile_pass[fr.tile0 + (ty0 + s) * fr.tiles_x + tx] = stamp;
 				work += (unsigned long long)(ncols * min(TILE_H, h - ty));
 				sh.na = 0;
 			}
@@ -1080,6 +1080,178 @@ significance_pass(SigShared &sh, const Params &p, int iter)
 	 * serialise the whole pass in L2 */
 	if (!threadIdx.x && work)
 		atomicAdd(&p.work[iter], work);
+}
+
+/* S at an in-frame pixel (x, y); bad = its noise value is invalid. */
+template <bool MODEL, bool EXACT, bool TAME>
+__device__ __forceinline__ double
+significance_at(const Params &p, const Frame &fr, int x, int y, bool &bad)
+{
+	const int mode = p.mode, w = fr.w, h = fr.h;
+	const double *clean = p.clean + fr.off;
+	double qc = clean[y * w + x] * 0.25;
+	double qu = lap_sample(clean, x, y - 1, w, h, mode) * 0.25;
+	double qd = lap_sample(clean, x, y + 1, w, h, mode) * 0.25;
+	double ql = lap_sample(clean, x - 1, y, w, h, mode) * 0.25;
+	double qr = lap_sample(clean, x + 1, y, w, h, mode) * 0.25;
+	double nz;
+	if (!MODEL || fr.error) {
+		nz = fr.error[y * w + x];
+	} else {
+		double v[25], m;
+#pragma unroll
+		for (int k = 0; k < 25; ++k)
+			v[k] = med_sample(clean, x - 2 + k % 5, y - 2 + k / 5, w, h, mode);
+		median25(v, &m);
+		nz = noise_model(m, fr.gain ? fr.gain[y * w + x] : fr.g,
+			fr.readnoise ? fr.readnoise[y * w + x] : fr.rn);
+	}
+	bad = invalid_noise(nz);
+	return laplacian_q<EXACT, TAME>(qc, qu, qd, ql, qr, w == 1) / (2 * nz);
+}
+
+constexpr int BR = BLK + 2 * HS; /* edge of the S region of a block */
+constexpr int WAVE_SEEDS = 256;  /* seeds buffered per wave between flushes */
+static_assert(BLK * BLK == 64, "one lane per pixel of a block");
+
+struct BlockShared {
+	double s[BLOCK / 64][BR * BR];
+	unsigned long long seeds[BLOCK / 64][WAVE_SEEDS];
+	int nseed[BLOCK / 64];
+};
+
+/* Entry e of the dirty list, as a wave-uniform value. */
+__device__ __forceinline__ unsigned long long
+dlist_entry(const Params &p, unsigned e)
+{
+	unsigned long long v = p.dirty_list[e];
+	unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+	unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+	return (unsigned long long)hi << 32 | lo;
+}
+
+/* Move this wave's buffered seeds to the global list (whole wave). */
+__device__ __forceinline__ void
+flush_wave_seeds(unsigned long long *buf, int *n, unsigned long long *list, unsigned int *count)
+{
+	const int lane = threadIdx.x & 63, k = *n;
+	unsigned base = 0;
+	if (!lane && k)
+		base = atomicAdd(count, (unsigned)k);
+	base = __shfl(base, 0, 64);
+	for (int j = lane; j < k; j += 64)
+		list[base + j] = buf[j];
+	__builtin_amdgcn_wave_barrier();
+	if (!lane)
+		*n = 0;
+	__builtin_amdgcn_wave_barrier();
+}
+
+/* Passes after the first: one wave per dirty block, independent of the other
+ * waves. S over the block and a 2-pixel halo goes to the wave's LDS region;
+ * a slot outside the frame holds S at its mapped in-frame pixel (median border
+ * mode; 0 for CONSTANT), which WRAP's periodic stencils also give. Screened
+ * pixels of the block get S' as in the first pass. */
+template <bool MODEL, bool EXACT, bool TAME>
+__device__ __forceinline__ void
+significance_blocks_pass(BlockShared &sh, const Params &p, int iter)
+{
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int mode = p.mode;
+	const double t_cr = p.threshold, t_min = fmin(p.threshold, p.neighbor);
+	const unsigned stamp = pass_stamp(iter);
+	const unsigned n = p.nlist[nl_dirty(p, iter)];
+	unsigned int *seeds = &p.nlist[nl_seed(p, iter)];
+	double *S = sh.s[wave];
+	unsigned long long *buf = sh.seeds[wave];
+	int *nseed = &sh.nseed[wave];
+	unsigned long long work = 0;
+	if (!lane)
+		*nseed = 0;
+	__builtin_amdgcn_wave_barrier();
+	const unsigned waves = gridDim.x * (BLOCK / 64);
+	for (unsigned e = blockIdx.x * (BLOCK / 64) + wave; e < n; e += waves) {
+		const unsigned long long entry = dlist_entry(p, e);
+		const int f = (int)(entry >> 32), b = (int)(entry & 0xffffffffu);
+		const Frame fr = p.frames[f];
+		const int lb = b - fr.block0, w = fr.w, h = fr.h;
+		const int X0 = lb % fr.blocks_x * BLK, Y0 = lb / fr.blocks_x * BLK;
+		if (!lane) {
+			p.block_pass[b] = stamp;
+			work += (unsigned long long)(min(BLK, w - X0) * min(BLK, h - Y0));
+		}
+		bool bad = false;
+		/* the region and its one-pixel ring lie inside the frame */
+		const bool inner = X0 - HS - 1 >= 0 && X0 + BLK + HS < w && Y0 - HS - 1 >= 0 && Y0 + BLK + HS < h;
+		if ((!MODEL || fr.error) && inner) {
+			const double *cb = p.clean + fr.off + (long long)(Y0 - HS) * w + (X0 - HS);
+			const double *eb = fr.error + (long long)(Y0 - HS) * w + (X0 - HS);
+#pragma unroll
+			for (int j = 0; j < 3; ++j) {
+				const int k = lane + 64 * j;
+				if (j < 2 || k < BR * BR) {
+					const int o = (k / BR) * w + k % BR;
+					double nz = eb[o];
+					double q = laplacian_q<EXACT, TAME>(cb[o] * 0.25, cb[o - w] * 0.25,
+						cb[o + w] * 0.25, cb[o - 1] * 0.25, cb[o + 1] * 0.25, false);
+					bad |= invalid_noise(nz);
+					S[k] = q / (2 * nz);
+				}
+			}
+		} else {
+			for (int k = lane; k < BR * BR; k += 64) {
+				int mx = border_index(X0 - HS + k % BR, w, mode);
+				int my = border_index(Y0 - HS + k / BR, h, mode);
+				double v = 0.0;
+				if (mx >= 0 && my >= 0) {
+					bool bk;
+					v = significance_at<MODEL, EXACT, TAME>(p, fr, mx, my, bk);
+					bad |= bk;
+				}
+				S[k] = v;
+			}
+		}
+		const bool unsafe = __ballot(bad) != 0;
+		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+		__builtin_amdgcn_wave_barrier();
+		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+		const int ox = lane % BLK, oy = lane / BLK, x = X0 + ox, y = Y0 + oy;
+		const int c = (oy + HS) * BR + ox + HS;
+		bool seed = false;
+		if (x < w && y < h && (unsafe || S[c] > t_min)) {
+			double v[25], m;
+#pragma unroll
+			for (int k = 0; k < 25; ++k)
+				v[k] = S[c + (k / 5 - 2) * BR + k % 5 - 2];
+			median25(v, &m);
+			double sp = v[12] - m;
+			long long g = fr.off + (long long)y * w + x;
+			p.sprime[g] = sp;
+			reinterpret_cast<uint16_t *>(p.flags + g)[1] = (uint16_t)stamp;
+			seed = sp > t_cr;
+		}
+		lds_append(seed, list_entry(f, y * w + x), buf, nseed);
+		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+		__builtin_amdgcn_wave_barrier();
+		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+		if (*nseed > WAVE_SEEDS - 64)
+			flush_wave_seeds(buf, nseed, p.list_b, seeds);
+	}
+	flush_wave_seeds(buf, nseed, p.list_b, seeds);
+	if (!lane && work)
+		atomicAdd(&p.work[iter], work);
+}
+
+template <bool MODEL, bool EXACT>
+__global__ __launch_bounds__(BLOCK) void
+significance_blocks(Params p, int iter)
+{
+	__shared__ BlockShared sh;
+	if (p.status[1])
+		significance_blocks_pass<MODEL, EXACT, false>(sh, p, iter);
+	else
+		significance_blocks_pass<MODEL, EXACT, true>(sh, p, iter);
 }
 
 /* Every frame has an error map. The pass is bound by load latency, so the
@@ -1245,15 +1417,16 @@ fine(Params p, int iter)
 	}
 }
 
-/* S' at pixel (x, y) of a frame if the last pass that processed its tile
- * computed it there (0 at holes, src/filters.cuh:137-146), else -inf: neither
- * can pass a threshold t >= 0. */
+/* S' at pixel (x, y) of a frame if the last pass that processed the pixel
+ * (the first pass, or a later one through its dirty block) computed it there
+ * (0 at holes, src/filters.cuh:137-146), else -inf: neither can pass a
+ * threshold t >= 0. */
 __device__ __forceinline__ double
-sprime_at(const Params &p, long long off, int tile0, int tiles_x, int x, int y, int w)
+sprime_at(const Params &p, long long off, int block0, int blocks_x, int x, int y, int w)
 {
 	long long g = off + (long long)y * w + x;
 	uint32_t fw = p.flags[g];
-	unsigned last = p.tile_pass[tile0 + (y / TILE_H) * tiles_x + x / TILE_W];
+	unsigned last = max(pass_stamp(0), p.block_pass[block0 + (y / BLK) * blocks_x + x / BLK]);
 	return (fw >> STAMP_SHIFT) == last && !(fw & HOLE) ? p.sprime[g] : -INFINITY;
 }
 
@@ -1269,7 +1442,7 @@ grow_first(Params p, int iter)
 		out.n = 0;
 	for (unsigned base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
 		unsigned e = base + threadIdx.x;
-		int f = 0, x = 0, y = 0, w = 1, h = 1, tile0 = 0, tiles_x = 1;
+		int f = 0, x = 0, y = 0, w = 1, h = 1, block0 = 0, blocks_x = 1;
 		long long off = 0;
 		if (e < n) {
 			unsigned long long entry = p.list_c[e];
@@ -1279,8 +1452,8 @@ grow_first(Params p, int iter)
 			w = fr.w;
 			h = fr.h;
 			off = fr.off;
-			tile0 = fr.tile0;
-			tiles_x = fr.tiles_x;
+			block0 = fr.block0;
+			blocks_x = fr.blocks_x;
 			x = i % w;
 			y = i / w;
 		}
@@ -1290,7 +1463,7 @@ grow_first(Params p, int iter)
 			bool join = false;
 			int q = qy * w + qx;
 			if (e < n && qx >= 0 && qx < w && qy >= 0 && qy < h &&
-			    sprime_at(p, off, tile0, tiles_x, qx, qy, w) > p.threshold)
+			    sprime_at(p, off, block0, blocks_x, qx, qy, w) > p.threshold)
 				join = !(atomicOr(&p.flags[off + q], GROW) & GROW);
 			lds_append(join, list_entry(f, q), out.item, &out.n);
 		}
@@ -1310,7 +1483,7 @@ grow_second(Params p, int iter)
 		out.n = 0;
 	for (unsigned base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
 		unsigned e = base + threadIdx.x;
-		int f = 0, x = 0, y = 0, w = 1, h = 1, tile0 = 0, tiles_x = 1, fresh_n = 0, lost_n = 0;
+		int f = 0, x = 0, y = 0, w = 1, h = 1, block0 = 0, blocks_x = 1, fresh_n = 0, lost_n = 0;
 		long long off = 0;
 		if (e < n) {
 			unsigned long long entry = p.list_b[e];
@@ -1320,8 +1493,8 @@ grow_second(Params p, int iter)
 			w = fr.w;
 			h = fr.h;
 			off = fr.off;
-			tile0 = fr.tile0;
-			tiles_x = fr.tiles_x;
+			block0 = fr.block0;
+			blocks_x = fr.blocks_x;
 			x = i % w;
 			y = i / w;
 			atomicAnd(&p.flags[off + i], ~GROW);
@@ -1332,7 +1505,7 @@ grow_second(Params p, int iter)
 			bool fresh = false;
 			int q = qy * w + qx;
 			if (e < n && qx >= 0 && qx < w && qy >= 0 && qy < h &&
-			    sprime_at(p, off, tile0, tiles_x, qx, qy, w) > p.neighbor) {
+			    sprime_at(p, off, block0, blocks_x, qx, qy, w) > p.neighbor) {
 				uint32_t old = atomicOr(&p.flags[off + q], CRAY);
 				if (!(old & CRAY)) {
 					fresh = true;
