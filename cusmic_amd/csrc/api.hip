@@ -250,7 +250,7 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	size_t n_counts = nf * (sizeof(Counts) / 4);
 	size_t n_todo = size_t(std::max(maxiter, 1)) * nf;
 	size_t n_queue = size_t(std::max(maxiter, 1));
-	size_t n_nlist = 3 + 6 * size_t(std::max(maxiter, 1));
+	size_t n_nlist = 4 + 9 * size_t(std::max(maxiter, 1));
 	size_t words = n_counts + n_todo + n_queue + n_nlist + 2 + 3 * size_t(nblocks);
 	size_t off_frames = 0;
 	size_t off_words = align_up(nf * sizeof(Frame), 256);
@@ -265,7 +265,8 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	size_t off_list = align_up(off_sprime + planes * 8, 256);
 	size_t off_list_b = align_up(off_list + planes * 8, 256);
 	size_t off_list_c = align_up(off_list_b + planes * 8, 256);
-	size_t bytes = off_list_c + planes * 8;
+	size_t off_wide = align_up(off_list_c + planes * 8, 256);
+	size_t bytes = off_wide + planes * 8;
 	scratch mem(bytes, stream);
 	char *base = static_cast<char *>(mem.ptr);
 	hip_check(hipMemcpyAsync(base + off_frames, frames.data(), nf * sizeof(Frame),
@@ -307,6 +308,7 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	p.list = reinterpret_cast<unsigned long long *>(base + off_list);
 	p.list_b = reinterpret_cast<unsigned long long *>(base + off_list_b);
 	p.list_c = reinterpret_cast<unsigned long long *>(base + off_list_c);
+	p.wide_all = reinterpret_cast<unsigned long long *>(base + off_wide);
 
 	unsigned table_grid = unsigned(std::min<long long>(((long long)nchunks + nruns + 255) / 256, 4096));
 	hipLaunchKernelGGL(set_tables, dim3(table_grid), dim3(256), 0, stream, p);
@@ -357,6 +359,8 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 		hipLaunchKernelGGL(fine, dim3(list_grid), dim3(BLOCK), 0, stream, p, it);
 		hipLaunchKernelGGL(grow_first, dim3(list_grid), dim3(BLOCK), 0, stream, p, it);
 		hipLaunchKernelGGL(grow_second, dim3(list_grid), dim3(BLOCK), 0, stream, p, it);
+		if (it > 0)
+			hipLaunchKernelGGL(redo_scan, dim3(list_grid), dim3(BLOCK), 0, stream, p, it);
 		tm.mark(CUSMIC_PROFILE_DETECT_MS);
 		hipLaunchKernelGGL(replace_pass, dim3(replace_grid), dim3(BLOCK), 0, stream, p, it, 0);
 		hipLaunchKernelGGL(replace_widened, dim3(cus), dim3(BLOCK), 0, stream, p, it, 0);

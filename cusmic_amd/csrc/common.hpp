@@ -38,6 +38,10 @@ constexpr uint32_t HOLE = 1;    /* input pixel is nonfinite */
 constexpr uint32_t EXCL = 2;    /* hole or masked: never a seed or donor */
 constexpr uint32_t CRAY = 4;    /* flagged as cosmic ray (the output mask) */
 constexpr uint32_t GROW = 8;    /* on this pass's first-growth list */
+constexpr uint32_t NEWP = 16;   /* flagged in this pass, not yet replaced */
+constexpr uint32_t REDO = 32;   /* older flagged pixel on this pass's redo list */
+constexpr uint32_t WIDE = 64;   /* on the list of pixels replaced from a
+				 * window wider than 5 x 5 */
 constexpr int STAMP_SHIFT = 16;
 
 constexpr double FINE_FLOOR = 0.01;   /* src/filters.cuh:21 */
@@ -78,7 +82,10 @@ struct Params {
 	uint8_t *mask;
 	double *sprime;                  /* S' where the stamp matches the pass */
 	unsigned long long *list_b;      /* seed candidates, then first-growth pixels */
-	unsigned long long *list_c;      /* candidates that passed the contrast test */
+	unsigned long long *list_c;      /* candidates that passed the contrast test,
+					  * then this pass's widened replacements */
+	unsigned long long *wide_all;    /* every flagged pixel replaced from a
+					  * window wider than 5 x 5 so far */
 	int32_t *donors;                 /* 2 per pixel or null */
 	unsigned long long *list;        /* hole list, then the flagged-pixel list */
 	Counts *counts;                  /* per frame */
@@ -88,7 +95,10 @@ struct Params {
 					  * [2 + iter] dirty blocks, [2 + M + iter] seeds,
 					  * [2 + 2M + iter] candidates, [2 + 3M + iter] grown,
 					  * [2 + 4M + iter] widened replacements (iter = M: fill),
-					  * [3 + 5M + iter] changed blocks */
+					  * [3 + 5M + iter] changed blocks, [3 + 6M + iter]
+					  * list length before the pass's flags,
+					  * [3 + 7M + iter] redo list, [3 + 8M + iter] wide_all
+					  * length before the pass, [3 + 9M] wide_all length */
 	unsigned int *changed_tag;       /* [nblocks] stamp of the last pass whose
 					  * preceding replacement changed a pixel of
 					  * the block */
@@ -133,6 +143,10 @@ __device__ __forceinline__ int nl_cand(const Params &p, int it) { return 2 + 2 *
 __device__ __forceinline__ int nl_grow(const Params &p, int it) { return 2 + 3 * p.maxiter + it; }
 __device__ __forceinline__ int nl_wide(const Params &p, int it) { return 2 + 4 * p.maxiter + it; }
 __device__ __forceinline__ int nl_changed(const Params &p, int it) { return 3 + 5 * p.maxiter + it; }
+__device__ __forceinline__ int nl_start(const Params &p, int it) { return 3 + 6 * p.maxiter + it; }
+__device__ __forceinline__ int nl_redo(const Params &p, int it) { return 3 + 7 * p.maxiter + it; }
+__device__ __forceinline__ int nl_wide_base(const Params &p, int it) { return 3 + 8 * p.maxiter + it; }
+__device__ __forceinline__ int nl_wide_all(const Params &p) { return 3 + 9 * p.maxiter; }
 
 /* scipy.ndimage border mapping of coordinate x on an axis of n samples;
  * -1 means a CONSTANT-mode zero (src/filters.cuh:24-44). */

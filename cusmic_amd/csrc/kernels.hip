@@ -646,20 +646,28 @@ replace_allowed(const Params &p, int f, int iter, int fill)
 }
 
 /* Hole fill before the loop (fill = 1, src/cleaner.cuh:78-100) or the
- * replacement pass of iteration iter over every flagged pixel. Pixels whose
- * 5x5 window has no donor go to the widened list. */
+ * replacement pass of iteration iter (src/cleaner.cuh:175-181). Donor values
+ * never change (replacement writes flagged pixels only), so a flagged pixel's
+ * value changes only when its window loses a donor: the pass replaces the
+ * pixels it flagged, then the older ones within 2 pixels of them (redo list);
+ * pixels that needed a window wider than 5 x 5 are redone by replace_widened.
+ * Pixels whose 5x5 window has no donor go to the widened list. */
 __global__ __launch_bounds__(BLOCK) void
 replace_pass(Params p, int iter, int fill)
 {
-	const unsigned n = fill ? p.nlist[1] : p.nlist[0];
+	const unsigned start = fill ? 0 : p.nlist[nl_start(p, iter)];
+	const unsigned n_new = fill ? p.nlist[1] : p.nlist[0] - start;
+	const unsigned n = n_new + (fill ? 0 : p.nlist[nl_redo(p, iter)]);
 	unsigned int *wide = &p.nlist[nl_wide(p, fill ? p.maxiter : iter)];
+	if (!fill && !blockIdx.x && !threadIdx.x)
+		p.nlist[nl_wide_base(p, iter)] = p.nlist[nl_wide_all(p)];
 	for (unsigned base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
 		unsigned e = base + threadIdx.x;
 		bool widen = false, changed = false;
 		unsigned long long entry = 0;
 		int f = 0, i = 0;
 		if (e < n) {
-			entry = p.list[e];
+			entry = e < n_new ? p.list[start + e] : p.list_b[e - n_new];
 			f = (int)(entry >> 32);
 			i = (int)(entry & 0xffffffffu);
 			if (replace_allowed(p, f, iter, fill)) {
@@ -667,6 +675,8 @@ replace_pass(Params p, int iter, int fill)
 				int r = replace_near(p, fr, i, fill ? nullptr : p.donors);
 				widen = r < 0;
 				changed = r > 0;
+				if (!fill)
+					atomicAnd(&p.flags[fr.off + i], ~(NEWP | REDO));
 			}
 		}
 		list_push(widen, entry, p.list_c, wide);
@@ -675,17 +685,74 @@ replace_pass(Params p, int iter, int fill)
 	}
 }
 
+/* Windows wider than 5 x 5: this pass's widened pixels, then (after the hole
+ * fill) every pixel widened in an earlier pass, whose window radius may have
+ * grown. A pixel widened for the first time joins wide_all. */
 __global__ __launch_bounds__(BLOCK) void
 replace_widened(Params p, int iter, int fill)
 {
-	const unsigned n = p.nlist[nl_wide(p, fill ? p.maxiter : iter)];
+	const unsigned n_pass = p.nlist[nl_wide(p, fill ? p.maxiter : iter)];
+	const unsigned n = n_pass + (fill ? 0 : p.nlist[nl_wide_base(p, iter)]);
 	for (unsigned e = blockIdx.x * BLOCK + threadIdx.x; e < n; e += gridDim.x * BLOCK) {
-		unsigned long long entry = p.list_c[e];
+		const bool in_pass = e < n_pass;
+		unsigned long long entry = in_pass ? p.list_c[e] : p.wide_all[e - n_pass];
 		int f = (int)(entry >> 32), i = (int)(entry & 0xffffffffu);
-		const Frame fr = p.frames[f];
-		bool changed = replace_wide(p, fr, i, fill ? nullptr : p.donors);
+		bool changed = false, join = false;
+		if (in_pass || replace_allowed(p, f, iter, 0)) {
+			const Frame fr = p.frames[f];
+			changed = replace_wide(p, fr, i, fill ? nullptr : p.donors);
+			if (!fill && in_pass)
+				join = !(atomicOr(&p.flags[fr.off + i], WIDE) & WIDE);
+		}
+		list_push(join, entry, p.wide_all, &p.nlist[nl_wide_all(p)]);
 		if (!fill && iter + 1 < p.maxiter)
 			mark_dirty(p, changed, f, i, iter + 1);
+	}
+}
+
+/* Older flagged pixels within 2 pixels of a pixel that pass iter flagged have
+ * lost a donor from their 5x5 window: each joins the pass's redo list once
+ * (REDO bit). Pixels flagged in this pass (NEWP) are replaced anyway and
+ * pixels on wide_all (WIDE) are redone anyway. */
+__global__ __launch_bounds__(BLOCK) void
+redo_scan(Params p, int iter)
+{
+	__shared__ BlockList<BLOCK * 8> out;
+	const unsigned start = p.nlist[nl_start(p, iter)], n = p.nlist[0] - start;
+	unsigned int *count = &p.nlist[nl_redo(p, iter)];
+	if (!threadIdx.x)
+		out.n = 0;
+	for (unsigned base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
+		const unsigned e = base + threadIdx.x;
+		int f = 0, x = 0, y = 0, w = 1, h = 1;
+		long long off = 0;
+		if (e < n) {
+			unsigned long long entry = p.list[start + e];
+			f = (int)(entry >> 32);
+			int i = (int)(entry & 0xffffffffu);
+			const Frame fr = p.frames[f];
+			w = fr.w;
+			h = fr.h;
+			off = fr.off;
+			x = i % w;
+			y = i / w;
+		}
+		/* 24 neighbours in rounds of 8, one buffer flush each */
+		for (int k0 = 0; k0 < 24; k0 += 8) {
+			__syncthreads();
+			for (int k = k0; k < k0 + 8; ++k) {
+				const int o = k < 12 ? k : k + 1; /* skip the centre */
+				const int qx = x + o % 5 - 2, qy = y + o / 5 - 2, q = qy * w + qx;
+				bool redo = false;
+				if (e < n && qx >= 0 && qx < w && qy >= 0 && qy < h) {
+					uint32_t fl = p.flags[off + q];
+					if ((fl & CRAY) && !(fl & (NEWP | WIDE | REDO)))
+						redo = !(atomicOr(&p.flags[off + q], REDO) & REDO);
+				}
+				lds_append(redo, list_entry(f, q), out.item, &out.n);
+			}
+			block_list_flush(out, p.list_b, count);
+		}
 	}
 }
 
@@ -1440,6 +1507,9 @@ grow_first(Params p, int iter)
 	unsigned int *count = &p.nlist[nl_grow(p, iter)];
 	if (!threadIdx.x)
 		out.n = 0;
+	/* the flagged list before this pass: its tail will be the new pixels */
+	if (!blockIdx.x && !threadIdx.x)
+		p.nlist[nl_start(p, iter)] = p.nlist[0];
 	for (unsigned base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
 		unsigned e = base + threadIdx.x;
 		int f = 0, x = 0, y = 0, w = 1, h = 1, block0 = 0, blocks_x = 1;
@@ -1508,6 +1578,7 @@ grow_second(Params p, int iter)
 			    sprime_at(p, off, block0, blocks_x, qx, qy, w) > p.neighbor) {
 				uint32_t old = atomicOr(&p.flags[off + q], CRAY);
 				if (!(old & CRAY)) {
+					atomicOr(&p.flags[off + q], NEWP);
 					fresh = true;
 					fresh_n += 1;
 					lost_n += !(old & EXCL) && isfinite(p.clean[off + q]);
