@@ -261,7 +261,8 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	size_t off_rframe = align_up(off_cframe + size_t(nchunks) * 4, 256);
 	size_t off_flags = align_up(off_rframe + size_t(nruns) * 4, 256);
 	size_t planes = maxiter ? total : 0;
-	size_t off_sprime = align_up(off_flags + planes * 4, 256);
+	size_t off_stamp = align_up(off_flags + planes, 256);
+	size_t off_sprime = align_up(off_stamp + planes * 2, 256);
 	size_t off_list = align_up(off_sprime + planes * 8, 256);
 	size_t off_list_b = align_up(off_list + planes * 8, 256);
 	size_t off_list_c = align_up(off_list_b + planes * 8, 256);
@@ -303,7 +304,8 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	p.run_frame = reinterpret_cast<int *>(base + off_rframe);
 	p.nruns = nruns;
 	p.nchunks = nchunks;
-	p.flags = reinterpret_cast<uint32_t *>(base + off_flags);
+	p.flags = reinterpret_cast<uint8_t *>(base + off_flags);
+	p.stamp = reinterpret_cast<uint16_t *>(base + off_stamp);
 	p.sprime = reinterpret_cast<double *>(base + off_sprime);
 	p.list = reinterpret_cast<unsigned long long *>(base + off_list);
 	p.list_b = reinterpret_cast<unsigned long long *>(base + off_list_b);
@@ -366,7 +368,14 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 		hipLaunchKernelGGL(replace_widened, dim3(cus), dim3(BLOCK), 0, stream, p, it, 0);
 		tm.mark(CUSMIC_PROFILE_REPLACE_MS);
 	}
-	hipLaunchKernelGGL(epilogue, dim3(nchunks), dim3(BLOCK), 0, stream, p);
+	{
+		int bg_maps = 0;
+		for (size_t f = 0; f < nf; ++f)
+			bg_maps |= ims[f].background != nullptr;
+		long long groups = ((long long)total + 7) / 8;
+		unsigned grid = unsigned(std::min<long long>((groups + BLOCK - 1) / BLOCK, (long long)cus * 16));
+		hipLaunchKernelGGL(epilogue, dim3(grid), dim3(BLOCK), 0, stream, p, (long long)total, bg_maps);
+	}
 	tm.mark(CUSMIC_PROFILE_EPILOGUE_MS);
 	hip_check(hipGetLastError());
 

@@ -29,11 +29,10 @@ constexpr int RUN_TILES = 8;
  * dirty by the previous replacement. */
 constexpr int BLK = 8;
 
-/* Per-pixel state word (u32) of the flags plane: bits 0-7 are flags, bits
- * 16-31 hold the stamp of the last pass that computed S' at the pixel. S' at a
- * pixel is current iff that stamp equals the stamp of the last pass that
- * processed the pixel: the first pass processes every pixel, later passes the
- * dirty blocks (Params::block_pass). */
+/* Per-pixel flags (u8). A separate u16 plane holds the stamp of the last pass
+ * that computed S' at the pixel: S' is current iff that stamp equals the stamp
+ * of the last pass that processed the pixel (the first pass processes every
+ * pixel, later passes the dirty blocks, Params::block_pass). */
 constexpr uint32_t HOLE = 1;    /* input pixel is nonfinite */
 constexpr uint32_t EXCL = 2;    /* hole or masked: never a seed or donor */
 constexpr uint32_t CRAY = 4;    /* flagged as cosmic ray (the output mask) */
@@ -42,7 +41,6 @@ constexpr uint32_t NEWP = 16;   /* flagged in this pass, not yet replaced */
 constexpr uint32_t REDO = 32;   /* older flagged pixel on this pass's redo list */
 constexpr uint32_t WIDE = 64;   /* on the list of pixels replaced from a
 				 * window wider than 5 x 5 */
-constexpr int STAMP_SHIFT = 16;
 
 constexpr double FINE_FLOOR = 0.01;   /* src/filters.cuh:21 */
 constexpr double SIGNAL_FLOOR = 1e-5; /* src/filters.cuh:22 */
@@ -78,9 +76,10 @@ struct Params {
 	int nf, ntiles, maxiter, mode, laplacian;
 	double contrast, threshold, neighbor, bg;  /* bg: scalar background or NaN */
 	double *clean;
-	uint32_t *flags;
+	uint8_t *flags;
+	uint16_t *stamp;                 /* S' stamps */
 	uint8_t *mask;
-	double *sprime;                  /* S' where the stamp matches the pass */
+	double *sprime;                  /* S' where the stamp is current */
 	unsigned long long *list_b;      /* seed candidates, then first-growth pixels */
 	unsigned long long *list_c;      /* candidates that passed the contrast test,
 					  * then this pass's widened replacements */
@@ -223,6 +222,24 @@ find_frame_of_block(const Frame *frames, int nf, int block)
 			hi = mid - 1;
 	}
 	return lo;
+}
+
+/* Atomic OR / AND-NOT of flag bits on one byte of the flags plane, through
+ * the aligned 32-bit word that holds it; return the byte's old value. */
+__device__ __forceinline__ uint32_t
+flags_or(uint8_t *flag, uint32_t bits)
+{
+	uintptr_t a = reinterpret_cast<uintptr_t>(flag);
+	int sh = int(a & 3) * 8;
+	return (atomicOr(reinterpret_cast<uint32_t *>(a & ~uintptr_t(3)), bits << sh) >> sh) & 0xffu;
+}
+
+__device__ __forceinline__ uint32_t
+flags_clear(uint8_t *flag, uint32_t bits)
+{
+	uintptr_t a = reinterpret_cast<uintptr_t>(flag);
+	int sh = int(a & 3) * 8;
+	return (atomicAnd(reinterpret_cast<uint32_t *>(a & ~uintptr_t(3)), ~(bits << sh)) >> sh) & 0xffu;
 }
 
 __device__ __forceinline__ unsigned long long

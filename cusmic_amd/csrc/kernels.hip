@@ -212,7 +212,8 @@ prologue(Params p)
 					c += 1 * b[k];
 				long long g = fr.off + i;
 				p.clean[g] = c;
-				p.flags[g] = (hole ? HOLE : 0u) | (allow ? 0u : EXCL);
+				p.flags[g] = uint8_t((hole ? HOLE : 0u) | (allow ? 0u : EXCL));
+				p.stamp[g] = 0;
 				if (p.donors) {
 					p.donors[2 * g] = -1;
 					p.donors[2 * g + 1] = -1;
@@ -264,33 +265,72 @@ copy_without_detection(Params p)
 	}
 }
 
-/* Epilogue (src/image.cuh:50-70): out = finite D ? C - bg : D (payload bits
- * kept), mask = flagged bit. */
-__global__ __launch_bounds__(BLOCK) void
-epilogue(Params p)
+/* Frame that holds batch pixel g, searching forward from frame f. */
+__device__ __forceinline__ int
+frame_from(const Frame *frames, int nf, int f, long long g)
 {
-	const int f = p.chunk_frame[blockIdx.x];
-	const Frame fr = p.frames[f];
-	const int n = fr.w * fr.h, start = (blockIdx.x - fr.chunk0) * CHUNK + threadIdx.x;
-	const bool bg_map = fr.background != nullptr, bg = bg_map || !isnan(p.bg);
-	for (int base = start; base < start + CHUNK; base += PER * BLOCK) {
-		uint32_t fl[PER];
-#pragma unroll
-		for (int k = 0; k < PER; ++k) {
-			int i = base + k * BLOCK;
-			fl[k] = i < n ? p.flags[fr.off + i] : 0u;
+	while (f + 1 < nf && frames[f + 1].off <= g)
+		++f;
+	return f;
+}
+
+/* Epilogue (src/image.cuh:50-70): out = finite D ? C - bg : D (payload bits
+ * kept), mask = flagged bit. Eight batch pixels per lane and step, so that
+ * the byte planes move in 64-bit accesses: the mask bytes come from one load
+ * of the flags plane and leave in one store (bytewise where the caller's mask
+ * is not 8-byte aligned). Holes and background maps take a per-pixel path. */
+__global__ __launch_bounds__(BLOCK) void
+epilogue(Params p, long long total, int bg_maps)
+{
+	constexpr unsigned long long LOW = 0x0101010101010101ull;
+	static_assert(HOLE == 1 && CRAY == 4, "bit positions used below");
+	const bool vec_mask = !(reinterpret_cast<uintptr_t>(p.mask) & 7);
+	const bool bg_scalar = !isnan(p.bg);
+	const long long groups = (total + 7) / 8;
+	for (long long G = blockIdx.x * (long long)BLOCK + threadIdx.x; G < groups;
+	     G += (long long)gridDim.x * BLOCK) {
+		const long long g0 = G * 8;
+		const bool full = g0 + 8 <= total;
+		unsigned long long fl = 0;
+		if (full) {
+			fl = *reinterpret_cast<const unsigned long long *>(p.flags + g0);
+		} else {
+			for (int k = 0; g0 + k < total; ++k)
+				fl |= (unsigned long long)p.flags[g0 + k] << (8 * k);
 		}
-#pragma unroll
-		for (int k = 0; k < PER; ++k) {
-			int i = base + k * BLOCK;
-			if (i >= n)
-				continue;
-			long long g = fr.off + i;
-			if (fl[k] & HOLE)
-				p.clean[g] = fr.data[i];
-			else if (bg)
-				p.clean[g] = p.clean[g] + -1 * (bg_map ? fr.background[i] : p.bg);
-			p.mask[g] = (fl[k] & CRAY) ? 1 : 0;
+		const unsigned long long m = (fl >> 2) & LOW;
+		if (full && vec_mask) {
+			*reinterpret_cast<unsigned long long *>(p.mask + g0) = m;
+		} else {
+			for (int k = 0; k < 8 && g0 + k < total; ++k)
+				p.mask[g0 + k] = uint8_t(m >> (8 * k));
+		}
+		if ((fl & LOW) || bg_maps) {
+			int f = 0;
+			{
+				int lo = 0, hi = p.nf - 1;
+				while (lo < hi) {
+					int mid = (lo + hi + 1) >> 1;
+					if (p.frames[mid].off <= g0)
+						lo = mid;
+					else
+						hi = mid - 1;
+				}
+				f = lo;
+			}
+			for (int k = 0; k < 8 && g0 + k < total; ++k) {
+				const long long g = g0 + k;
+				f = frame_from(p.frames, p.nf, f, g);
+				const Frame &fr = p.frames[f];
+				const long long i = g - fr.off;
+				if ((fl >> (8 * k)) & HOLE)
+					p.clean[g] = fr.data[i];
+				else if (fr.background || bg_scalar)
+					p.clean[g] = p.clean[g] + -1 * (fr.background ? fr.background[i] : p.bg);
+			}
+		} else if (bg_scalar) {
+			for (int k = 0; k < 8 && g0 + k < total; ++k)
+				p.clean[g0 + k] = p.clean[g0 + k] + -1 * p.bg;
 		}
 	}
 }
@@ -299,7 +339,7 @@ epilogue(Params p)
 /* Replacement (src/replace.cuh:112-153)                                     */
 
 __device__ __forceinline__ bool
-is_donor(const uint32_t *flags, const double *clean, int q, double &v)
+is_donor(const uint8_t *flags, const double *clean, int q, double &v)
 {
 	if (flags[q] & (EXCL | CRAY))
 		return false;
@@ -311,7 +351,7 @@ is_donor(const uint32_t *flags, const double *clean, int q, double &v)
  * search over the 64 key bits: each step counts donors below the candidate
  * prefix. Used only when the 5x5 window holds no donor. */
 __device__ unsigned long long
-window_select(const uint32_t *flags, const double *clean, int w, int x0, int x1, int y0, int y1,
+window_select(const uint8_t *flags, const double *clean, int w, int x0, int x1, int y0, int y1,
 	int rank)
 {
 	unsigned long long prefix = 0;
@@ -338,7 +378,7 @@ window_select(const uint32_t *flags, const double *clean, int w, int x0, int x1,
 /* Index of the element of rank r in the (key, pixel index) order, given its
  * key: skip r - (#keys below) equal keys in raster order. */
 __device__ int
-rank_index(const uint32_t *flags, const double *clean, int w, int x0, int x1, int y0, int y1,
+rank_index(const uint8_t *flags, const double *clean, int w, int x0, int x1, int y0, int y1,
 	unsigned long long key, int r)
 {
 	int less = 0;
@@ -461,7 +501,7 @@ store_replacement(const Params &p, const Frame &fr, int i, int32_t *donors, int 
 	double hi, int neg0, int x0, int x1, int y0, int y1)
 {
 	const double *clean = p.clean + fr.off;
-	const uint32_t *flags = p.flags + fr.off;
+	const uint8_t *flags = p.flags + fr.off;
 	double out = replacement(n, lo, hi, neg0);
 	if (donors) {
 		int a = (n - 1) >> 1, b = n >> 1;
@@ -486,7 +526,7 @@ gather_window(const Params &p, const Frame &fr, int x, int y, double (&v)[K], in
 {
 	constexpr int S = 2 * R + 1, C = R * S + R;
 	const double *clean = p.clean + fr.off;
-	const uint32_t *flags = p.flags + fr.off;
+	const uint8_t *flags = p.flags + fr.off;
 	const int w = fr.w, h = fr.h, i = y * w + x;
 	uint32_t fl[K];
 #pragma unroll
@@ -558,7 +598,7 @@ __device__ bool
 replace_wide(const Params &p, const Frame &fr, int i, int32_t *donors)
 {
 	const double *clean = p.clean + fr.off;
-	const uint32_t *flags = p.flags + fr.off;
+	const uint8_t *flags = p.flags + fr.off;
 	const int w = fr.w, h = fr.h, x = i % w, y = i / w;
 	{
 		double v[48], s[25];
@@ -676,7 +716,7 @@ replace_pass(Params p, int iter, int fill)
 				widen = r < 0;
 				changed = r > 0;
 				if (!fill)
-					atomicAnd(&p.flags[fr.off + i], ~(NEWP | REDO));
+					flags_clear(&p.flags[fr.off + i], NEWP | REDO);
 			}
 		}
 		list_push(widen, entry, p.list_c, wide);
@@ -702,7 +742,7 @@ replace_widened(Params p, int iter, int fill)
 			const Frame fr = p.frames[f];
 			changed = replace_wide(p, fr, i, fill ? nullptr : p.donors);
 			if (!fill && in_pass)
-				join = !(atomicOr(&p.flags[fr.off + i], WIDE) & WIDE);
+				join = !(flags_or(&p.flags[fr.off + i], WIDE) & WIDE);
 		}
 		list_push(join, entry, p.wide_all, &p.nlist[nl_wide_all(p)]);
 		if (!fill && iter + 1 < p.maxiter)
@@ -747,7 +787,7 @@ redo_scan(Params p, int iter)
 				if (e < n && qx >= 0 && qx < w && qy >= 0 && qy < h) {
 					uint32_t fl = p.flags[off + q];
 					if ((fl & CRAY) && !(fl & (NEWP | WIDE | REDO)))
-						redo = !(atomicOr(&p.flags[off + q], REDO) & REDO);
+						redo = !(flags_or(&p.flags[off + q], REDO) & REDO);
 				}
 				lds_append(redo, list_entry(f, q), out.item, &out.n);
 			}
@@ -1135,7 +1175,7 @@ significance_pass(SigShared &sh, const Params &p, int iter)
 					median25(v, &m);
 					double sp = v[12] - m;
 					p.sprime[fr.off + i] = sp;
-					reinterpret_cast<uint16_t *>(p.flags + fr.off + i)[1] = (uint16_t)stamp;
+					p.stamp[fr.off + i] = (uint16_t)stamp;
 					seed = sp > t_cr;
 				}
 				lds_append(seed, list_entry(f, i), sh.seeds, &sh.nseed);
@@ -1187,11 +1227,10 @@ struct BlockShared {
 	int nseed[BLOCK / 64];
 };
 
-/* Entry e of the dirty list, as a wave-uniform value. */
+/* A 64-bit value that is the same in every lane, as a wave-uniform value. */
 __device__ __forceinline__ unsigned long long
-dlist_entry(const Params &p, unsigned e)
+uniform_u64(unsigned long long v)
 {
-	unsigned long long v = p.dirty_list[e];
 	unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
 	unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
 	return (unsigned long long)hi << 32 | lo;
@@ -1237,8 +1276,13 @@ significance_blocks_pass(BlockShared &sh, const Params &p, int iter)
 		*nseed = 0;
 	__builtin_amdgcn_wave_barrier();
 	const unsigned waves = gridDim.x * (BLOCK / 64);
-	for (unsigned e = blockIdx.x * (BLOCK / 64) + wave; e < n; e += waves) {
-		const unsigned long long entry = dlist_entry(p, e);
+	unsigned e = blockIdx.x * (BLOCK / 64) + wave;
+	/* the next block's list entry is loaded while this one is processed */
+	unsigned long long next = e < n ? p.dirty_list[e] : 0;
+	for (; e < n; e += waves) {
+		const unsigned long long entry = uniform_u64(next);
+		if (e + waves < n)
+			next = p.dirty_list[e + waves];
 		const int f = (int)(entry >> 32), b = (int)(entry & 0xffffffffu);
 		const Frame fr = p.frames[f];
 		const int lb = b - fr.block0, w = fr.w, h = fr.h;
@@ -1295,7 +1339,7 @@ significance_blocks_pass(BlockShared &sh, const Params &p, int iter)
 			double sp = v[12] - m;
 			long long g = fr.off + (long long)y * w + x;
 			p.sprime[g] = sp;
-			reinterpret_cast<uint16_t *>(p.flags + g)[1] = (uint16_t)stamp;
+			p.stamp[g] = (uint16_t)stamp;
 			seed = sp > t_cr;
 		}
 		lds_append(seed, list_entry(f, y * w + x), buf, nseed);
@@ -1493,8 +1537,9 @@ sprime_at(const Params &p, long long off, int block0, int blocks_x, int x, int y
 {
 	long long g = off + (long long)y * w + x;
 	uint32_t fw = p.flags[g];
+	unsigned st = p.stamp[g];
 	unsigned last = max(pass_stamp(0), p.block_pass[block0 + (y / BLK) * blocks_x + x / BLK]);
-	return (fw >> STAMP_SHIFT) == last && !(fw & HOLE) ? p.sprime[g] : -INFINITY;
+	return st == last && !(fw & HOLE) ? p.sprime[g] : -INFINITY;
 }
 
 /* First growth (src/filters.cuh:177-193 at t): every pixel next to a
@@ -1534,7 +1579,7 @@ grow_first(Params p, int iter)
 			int q = qy * w + qx;
 			if (e < n && qx >= 0 && qx < w && qy >= 0 && qy < h &&
 			    sprime_at(p, off, block0, blocks_x, qx, qy, w) > p.threshold)
-				join = !(atomicOr(&p.flags[off + q], GROW) & GROW);
+				join = !(flags_or(&p.flags[off + q], GROW) & GROW);
 			lds_append(join, list_entry(f, q), out.item, &out.n);
 		}
 		block_list_flush(out, p.list_b, count);
@@ -1567,7 +1612,7 @@ grow_second(Params p, int iter)
 			blocks_x = fr.blocks_x;
 			x = i % w;
 			y = i / w;
-			atomicAnd(&p.flags[off + i], ~GROW);
+			flags_clear(&p.flags[off + i], GROW);
 		}
 		__syncthreads();
 		for (int k = 0; k < 9; ++k) {
@@ -1576,9 +1621,9 @@ grow_second(Params p, int iter)
 			int q = qy * w + qx;
 			if (e < n && qx >= 0 && qx < w && qy >= 0 && qy < h &&
 			    sprime_at(p, off, block0, blocks_x, qx, qy, w) > p.neighbor) {
-				uint32_t old = atomicOr(&p.flags[off + q], CRAY);
+				uint32_t old = flags_or(&p.flags[off + q], CRAY);
 				if (!(old & CRAY)) {
-					atomicOr(&p.flags[off + q], NEWP);
+					flags_or(&p.flags[off + q], NEWP);
 					fresh = true;
 					fresh_n += 1;
 					lost_n += !(old & EXCL) && isfinite(p.clean[off + q]);
