@@ -1426,21 +1426,30 @@ fine_structure(const Params &p, const Frame &fr, int x, int y)
 	 * position equals med3 at its mapped position; CONSTANT positions outside
 	 * the frame are 0; NEAREST near the edge takes the clamped position */
 	const bool block = inside || mode != CUSMIC_NEAREST;
+	/* the 9 x 9 block of C lies inside the frame: one row pointer */
+	const bool direct = x >= 4 && x + 4 < w && y >= 4 && y + 4 < h;
+	const double *rp = clean + (long long)(y - 4) * w + (x - 4);
 	double acc[2][22], a[9], b[9], c[9], centre = 0.0;
 	if (block) {
 #pragma unroll
 		for (int i = 0; i < 9; ++i) {
-			a[i] = med_sample(clean, x - 4 + i, y - 4, w, h, mode);
-			b[i] = med_sample(clean, x - 4 + i, y - 3, w, h, mode);
+			a[i] = direct ? rp[i] : med_sample(clean, x - 4 + i, y - 4, w, h, mode);
+			b[i] = direct ? rp[w + i] : med_sample(clean, x - 4 + i, y - 3, w, h, mode);
 		}
 	}
 #pragma unroll
 	for (int j = 0; j < 7; ++j) {
 		double row[7];
 		if (block) {
+			if (direct) {
 #pragma unroll
-			for (int i = 0; i < 9; ++i)
-				c[i] = med_sample(clean, x - 4 + i, y - 2 + j, w, h, mode);
+				for (int i = 0; i < 9; ++i)
+					c[i] = rp[(long long)(j + 2) * w + i];
+			} else {
+#pragma unroll
+				for (int i = 0; i < 9; ++i)
+					c[i] = med_sample(clean, x - 4 + i, y - 2 + j, w, h, mode);
+			}
 			/* 3x3 medians from sorted columns: med9 = med3(max of minima,
 			 * med3 of middles, min of maxima) */
 			double lo[9], md[9], hi[9];

@@ -165,6 +165,35 @@ def test_maps_and_background(cuda):
     assert_same_pixels(np_of(c), wo[0])
 
 
+@pytest.mark.parametrize("mode", MODES)
+def test_invalid_noise_values(cuda, mode):
+    """Device path without content checks: negative error values make S
+    negative, so the S' <= S screen must give way around them (first pass
+    and the dirty-block passes)."""
+    d, err, m = scene((120, 150), 21, nan=1e-3, mask_frac=2e-3)
+    rng = np.random.default_rng(21)
+    bad = rng.random(err.shape) < 5e-3
+    err = err.copy()
+    err[bad] = -err[bad]
+    say("invalid noise values", mode)
+    got, want = compare([dict(data=d, error=err, mask=m)], border_mode=mode, record_donors=True)
+    assert want[1][0].sum() > 20
+
+
+def test_many_passes(cuda):
+    """Low thresholds keep frames growing for many passes: the incremental
+    replacement, dirty blocks and stamps must track the dense loop."""
+    frames = []
+    for i, shape in enumerate(((96, 130), (65, 200), (150, 64))):
+        d, err, m = scene(shape, 30 + i, nan=1e-3, mask_frac=2e-3, stars=20, tracks=60, max_len=10)
+        frames.append(dict(data=d, error=err, mask=m))
+    for mode in ("mirror", "wrap", "constant"):
+        say("many passes", mode)
+        got, want = compare(frames, contrast=1.0, cr_threshold=3.0, neighbor_threshold=1.5,
+                            maxiter=12, border_mode=mode, record_donors=True)
+        assert max(want[3]) > 4, want[3]
+
+
 def test_tiny_and_odd_shapes(cuda):
     rng = np.random.default_rng(2)
     for shape in ((1, 9), (9, 1), (1, 1), (2, 2), (3, 70), (70, 3), (7, 7), (33, 65), (65, 33)):
