@@ -37,7 +37,7 @@ constexpr uint32_t HOLE = 1;    /* input pixel is nonfinite */
 constexpr uint32_t EXCL = 2;    /* hole or masked: never a seed or donor */
 constexpr uint32_t CRAY = 4;    /* flagged as cosmic ray (the output mask) */
 constexpr uint32_t GROW = 8;    /* on this pass's first-growth list */
-constexpr uint32_t NEWP = 16;   /* flagged in this pass, not yet replaced */
+constexpr uint32_t NEWP = 16;   /* flagged in the latest detection pass */
 constexpr uint32_t REDO = 32;   /* older flagged pixel on this pass's redo list */
 constexpr uint32_t WIDE = 64;   /* on the list of pixels replaced from a
 				 * window wider than 5 x 5 */

@@ -445,6 +445,16 @@ expand_dirty(Params p, int iter)
 	unsigned int *count = &p.nlist[nl_dirty(p, iter)];
 	if (!threadIdx.x)
 		out.n = 0;
+	/* the previous pass's new pixels stop being new (NEWP served its
+	 * replacement) */
+	{
+		const unsigned first = p.nlist[nl_start(p, iter - 1)], last = p.nlist[0];
+		for (unsigned e = first + blockIdx.x * BLOCK + threadIdx.x; e < last; e += gridDim.x * BLOCK) {
+			const unsigned long long entry = p.list[e];
+			const Frame &fr = p.frames[entry >> 32];
+			flags_clear(&p.flags[fr.off + (entry & 0xffffffffu)], NEWP);
+		}
+	}
 	for (unsigned base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
 		const unsigned e = base + threadIdx.x;
 		int f = 0, bx = 0, by = 0, nbx = 1, nby = 1, block0 = 0;
@@ -522,7 +532,8 @@ store_replacement(const Params &p, const Frame &fr, int i, int32_t *donors, int 
  * lane may be rewriting, is discarded. */
 template <int R, int K>
 __device__ __forceinline__ int
-gather_window(const Params &p, const Frame &fr, int x, int y, double (&v)[K], int &neg0)
+gather_window(const Params &p, const Frame &fr, int x, int y, double (&v)[K], int &neg0,
+	bool *any_new = nullptr)
 {
 	constexpr int S = 2 * R + 1, C = R * S + R;
 	const double *clean = p.clean + fr.off;
@@ -539,6 +550,13 @@ gather_window(const Params &p, const Frame &fr, int x, int y, double (&v)[K], in
 	}
 	int n = 0;
 	neg0 = 0;
+	if (any_new) {
+		bool a = false;
+#pragma unroll
+		for (int k = 0; k < K; ++k)
+			a |= (fl[k] & NEWP) != 0;
+		*any_new = a;
+	}
 #pragma unroll
 	for (int k = 0; k < K; ++k) {
 		bool ok = !(fl[k] & (EXCL | CRAY)) && isfinite(v[k]);
@@ -572,30 +590,14 @@ replace_near(const Params &p, const Frame &fr, int i, int32_t *donors)
 		max(0, y - 2), min(h - 1, y + 2));
 }
 
-/* rank-th smallest of n keys in a private array (MSB-first bit search) */
-__device__ unsigned long long
-array_select(const unsigned long long *keys, int n, int rank)
-{
-	unsigned long long prefix = 0;
-	for (int bit = 63; bit >= 0; --bit) {
-		unsigned long long hi_mask = bit == 63 ? 0 : ~0ull << (bit + 1);
-		int below = 0;
-		for (int j = 0; j < n; ++j)
-			below += (keys[j] & hi_mask) == (prefix & hi_mask) && !((keys[j] >> bit) & 1);
-		if (rank >= below) {
-			prefix |= 1ull << bit;
-			rank -= below;
-		}
-	}
-	return prefix;
-}
-
-constexpr int WIDE_KEYS = 288; /* donors of windows up to 17 x 17 */
 
 /* Replacement from a widened window, r = 3, 4, ... (a 5x5 window without
- * donors). Returns true if the pixel changed. */
+ * donors). Returns true if the pixel changed. With only_if_new, the pixel is
+ * left alone unless its final window holds a pixel flagged in this pass
+ * (NEWP): donors only disappear, so its previous window and donors are then
+ * unchanged. */
 __device__ bool
-replace_wide(const Params &p, const Frame &fr, int i, int32_t *donors)
+replace_wide(const Params &p, const Frame &fr, int i, int32_t *donors, bool only_if_new)
 {
 	const double *clean = p.clean + fr.off;
 	const uint8_t *flags = p.flags + fr.off;
@@ -603,8 +605,11 @@ replace_wide(const Params &p, const Frame &fr, int i, int32_t *donors)
 	{
 		double v[48], s[25];
 		int neg0;
-		int n = gather_window<3, 48>(p, fr, x, y, v, neg0);
+		bool any_new = false;
+		int n = gather_window<3, 48>(p, fr, x, y, v, neg0, &any_new);
 		if (n) {
+			if (only_if_new && !any_new)
+				return false;
 			sort_low25_of48(v, s);
 			int a = (n - 1) >> 1, b = n >> 1;
 			double lo = s[0], hi = s[0];
@@ -634,32 +639,184 @@ replace_wide(const Params &p, const Frame &fr, int i, int32_t *donors)
 	}
 	if (!n)
 		return false; /* unreachable: callers check the frame has donors */
-	double lo, hi;
-	int neg0 = 0;
-	if (n <= WIDE_KEYS) {
-		unsigned long long keys[WIDE_KEYS];
-		int k = 0;
-		for (int yy = y0; yy <= y1; ++yy)
-			for (int xx = x0; xx <= x1; ++xx) {
-				double t;
-				if (is_donor(flags, clean, yy * w + xx, t)) {
-					keys[k++] = order_key(t);
-					neg0 += t < 0 || (t == 0.0 && signbit(t));
-				}
-			}
-		lo = key_value(array_select(keys, n, (n - 1) >> 1));
-		hi = n & 1 ? lo : key_value(array_select(keys, n, n >> 1));
-	} else {
-		lo = key_value(window_select(flags, clean, w, x0, x1, y0, y1, (n - 1) >> 1));
-		hi = n & 1 ? lo : key_value(window_select(flags, clean, w, x0, x1, y0, y1, n >> 1));
-		for (int yy = y0; yy <= y1; ++yy)
-			for (int xx = x0; xx <= x1; ++xx) {
-				double t;
-				if (is_donor(flags, clean, yy * w + xx, t))
-					neg0 += t < 0 || (t == 0.0 && signbit(t));
-			}
+	if (only_if_new) {
+		bool any_new = false;
+		for (int yy = y0; yy <= y1 && !any_new; ++yy)
+			for (int xx = x0; xx <= x1; ++xx)
+				any_new |= (flags[yy * w + xx] & NEWP) != 0;
+		if (!any_new)
+			return false;
 	}
+	/* windows this wide are rare: the select reads the window from memory */
+	double lo = key_value(window_select(flags, clean, w, x0, x1, y0, y1, (n - 1) >> 1));
+	double hi = n & 1 ? lo : key_value(window_select(flags, clean, w, x0, x1, y0, y1, n >> 1));
+	int neg0 = 0;
+	for (int yy = y0; yy <= y1; ++yy)
+		for (int xx = x0; xx <= x1; ++xx) {
+			double t;
+			if (is_donor(flags, clean, yy * w + xx, t))
+				neg0 += t < 0 || (t == 0.0 && signbit(t));
+		}
 	return store_replacement(p, fr, i, donors, n, lo, hi, neg0, x0, x1, y0, y1);
+}
+
+/* A 64-bit value that is the same in every lane, as a wave-uniform value. */
+__device__ __forceinline__ unsigned long long
+uniform_u64(unsigned long long v)
+{
+	unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+	unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+	return (unsigned long long)hi << 32 | lo;
+}
+
+/* Window radius a wave handles in registers: the (2 RW + 1)^2 neighbourhood
+ * in chunks of 64 positions. */
+constexpr int RW = 10;
+constexpr int RW_CHUNKS = ((2 * RW + 1) * (2 * RW + 1) + 63) / 64;
+
+__device__ __forceinline__ int
+wave_sum(int v)
+{
+	for (int o = 32; o; o >>= 1)
+		v += __shfl_xor(v, o, 64);
+	return v;
+}
+
+/* Donors of a window whose donors all lie on its rim (at most 8 RW keys):
+ * compacted into RING_CHUNKS chunks of 64, unused entries at the maximum key. */
+constexpr int RING_CHUNKS = (8 * RW + 63) / 64;
+
+/* rank-th smallest of the keys held in the lanes' chunks, by MSB-first bit
+ * search: each step counts the keys under the candidate prefix whose next bit
+ * is 0 (ballot + popcount per chunk). */
+__device__ __forceinline__ unsigned long long
+wave_select(const unsigned long long (&key)[RING_CHUNKS], int rank)
+{
+	unsigned long long prefix = 0;
+	for (int bit = 63; bit >= 0; --bit) {
+		const unsigned long long hi_mask = bit == 63 ? 0 : ~0ull << (bit + 1);
+		int below = 0;
+#pragma unroll
+		for (int c = 0; c < RING_CHUNKS; ++c)
+			below += __popcll(__ballot((key[c] & hi_mask) == (prefix & hi_mask) && !((key[c] >> bit) & 1)));
+		if (rank >= below) {
+			prefix |= 1ull << bit;
+			rank -= below;
+		}
+	}
+	return prefix;
+}
+
+/* Replacement of pixel i from its smallest window wider than 5 x 5 that holds
+ * donors (src/replace.cuh:118-150, r = 3, 4, ...), by one whole wave: the
+ * neighbourhood up to RW is loaded at once, r is the smallest Chebyshev
+ * distance of a donor (the window's donors are exactly those at distance r),
+ * and the middle keys come from a wave radix select. With only_if_new the
+ * pixel is left alone unless its window holds a pixel flagged in this pass
+ * (NEWP): donors only disappear, so its window and donors are then unchanged.
+ * Larger windows fall back to replace_wide on one lane. Returns whether the
+ * pixel changed (the same in every lane). */
+__device__ bool
+replace_wide_wave(const Params &p, const Frame &fr, int i, int32_t *donors, bool only_if_new,
+	unsigned long long (*ring_keys)[64 * RING_CHUNKS])
+{
+	const int lane = threadIdx.x & 63;
+	const double *clean = p.clean + fr.off;
+	const uint8_t *flags = p.flags + fr.off;
+	const int w = fr.w, h = fr.h, x = i % w, y = i / w;
+	const int x0 = max(0, x - RW), y0 = max(0, y - RW);
+	const int side = min(w - 1, x + RW) - x0 + 1, npos = side * (min(h - 1, y + RW) - y0 + 1);
+	unsigned long long key[RW_CHUNKS];
+	int dist[RW_CHUNKS];
+	bool don[RW_CHUNKS], fresh[RW_CHUNKS];
+	int rmin = RW + 1;
+#pragma unroll
+	for (int c = 0; c < RW_CHUNKS; ++c) {
+		const int k = 64 * c + lane;
+		don[c] = fresh[c] = false;
+		dist[c] = RW + 1;
+		key[c] = 0;
+		if (k < npos) {
+			const int xx = x0 + k % side, yy = y0 + k / side, q = yy * w + xx;
+			const uint8_t fl = flags[q];
+			const double v = clean[q];
+			dist[c] = max(abs(xx - x), abs(yy - y));
+			don[c] = !(fl & (EXCL | CRAY)) && isfinite(v);
+			fresh[c] = (fl & NEWP) != 0;
+			key[c] = order_key(v);
+			if (don[c])
+				rmin = min(rmin, dist[c]);
+		}
+	}
+	for (int o = 32; o; o >>= 1)
+		rmin = min(rmin, __shfl_xor(rmin, o, 64));
+	if (rmin > RW) {
+		/* no donor within RW: the lane search over growing windows */
+		bool changed = false;
+		if (!lane)
+			changed = replace_wide(p, fr, i, donors, only_if_new);
+		return __shfl(changed ? 1 : 0, 0, 64) != 0;
+	}
+	const int r = rmin;
+	bool any_new = false;
+	int n = 0, neg0 = 0;
+#pragma unroll
+	for (int c = 0; c < RW_CHUNKS; ++c) {
+		any_new |= __ballot(fresh[c] && dist[c] <= r) != 0;
+		/* every donor within r is at distance exactly r */
+		n += don[c] && dist[c] == r;
+		const double v = key_value(key[c]);
+		neg0 += don[c] && dist[c] == r && (v < 0 || (v == 0.0 && signbit(v)));
+	}
+	if (only_if_new && !any_new)
+		return false;
+	n = wave_sum(n);
+	neg0 = wave_sum(neg0);
+	/* compact the rim donors (n <= 8r) into the wave's LDS slots */
+	unsigned long long *slot = ring_keys[threadIdx.x >> 6];
+	int base = 0;
+#pragma unroll
+	for (int c = 0; c < RW_CHUNKS; ++c) {
+		const bool sel = don[c] && dist[c] == r;
+		const unsigned long long m = __ballot(sel);
+		if (sel)
+			slot[base + __popcll(m & ((1ull << lane) - 1))] = key[c];
+		base += __popcll(m);
+	}
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+	unsigned long long rk[RING_CHUNKS];
+#pragma unroll
+	for (int c = 0; c < RING_CHUNKS; ++c) {
+		const int k = 64 * c + lane;
+		rk[c] = k < n ? slot[k] : ~0ull;
+	}
+	__builtin_amdgcn_wave_barrier();
+	const int a = (n - 1) >> 1;
+	const unsigned long long klo = wave_select(rk, a);
+	/* the upper middle: the same key if more than a + 1 keys are <= it, else
+	 * the smallest key above it */
+	unsigned long long khi = klo;
+	if (!(n & 1)) {
+		int le = 0;
+		unsigned long long above = ~0ull;
+#pragma unroll
+		for (int c = 0; c < RING_CHUNKS; ++c) {
+			le += __popcll(__ballot(64 * c + lane < n && rk[c] <= klo));
+			if (64 * c + lane < n && rk[c] > klo)
+				above = min(above, rk[c]);
+		}
+		for (int o = 32; o; o >>= 1)
+			above = min(above, (unsigned long long)__shfl_xor((long long)above, o, 64));
+		khi = le > a + 1 ? klo : above;
+	}
+	const double lo = key_value(klo), hi = key_value(khi);
+	bool changed = false;
+	if (!lane)
+		changed = store_replacement(p, fr, i, donors, n, lo, hi, neg0, max(0, x - r),
+			min(w - 1, x + r), max(0, y - r), min(h - 1, y + r));
+	return __shfl(changed ? 1 : 0, 0, 64) != 0;
 }
 
 /* May the pixels of frame f be replaced in this pass? Sets the no-donor error. */
@@ -716,7 +873,7 @@ replace_pass(Params p, int iter, int fill)
 				widen = r < 0;
 				changed = r > 0;
 				if (!fill)
-					flags_clear(&p.flags[fr.off + i], NEWP | REDO);
+					flags_clear(&p.flags[fr.off + i], REDO);
 			}
 		}
 		list_push(widen, entry, p.list_c, wide);
@@ -725,28 +882,31 @@ replace_pass(Params p, int iter, int fill)
 	}
 }
 
-/* Windows wider than 5 x 5: this pass's widened pixels, then (after the hole
- * fill) every pixel widened in an earlier pass, whose window radius may have
- * grown. A pixel widened for the first time joins wide_all. */
+/* Windows wider than 5 x 5, one wave per pixel: this pass's widened pixels,
+ * then (after the hole fill) every pixel widened in an earlier pass, whose
+ * window may have lost donors. A pixel widened for the first time joins
+ * wide_all. */
 __global__ __launch_bounds__(BLOCK) void
 replace_widened(Params p, int iter, int fill)
 {
+	__shared__ unsigned long long ring_keys[BLOCK / 64][64 * RING_CHUNKS];
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	const unsigned n_pass = p.nlist[nl_wide(p, fill ? p.maxiter : iter)];
 	const unsigned n = n_pass + (fill ? 0 : p.nlist[nl_wide_base(p, iter)]);
-	for (unsigned e = blockIdx.x * BLOCK + threadIdx.x; e < n; e += gridDim.x * BLOCK) {
+	for (unsigned e = blockIdx.x * (BLOCK / 64) + wave; e < n; e += gridDim.x * (BLOCK / 64)) {
 		const bool in_pass = e < n_pass;
-		unsigned long long entry = in_pass ? p.list_c[e] : p.wide_all[e - n_pass];
-		int f = (int)(entry >> 32), i = (int)(entry & 0xffffffffu);
+		const unsigned long long entry = uniform_u64(in_pass ? p.list_c[e] : p.wide_all[e - n_pass]);
+		const int f = (int)(entry >> 32), i = (int)(entry & 0xffffffffu);
 		bool changed = false, join = false;
 		if (in_pass || replace_allowed(p, f, iter, 0)) {
 			const Frame fr = p.frames[f];
-			changed = replace_wide(p, fr, i, fill ? nullptr : p.donors);
-			if (!fill && in_pass)
+			changed = replace_wide_wave(p, fr, i, fill ? nullptr : p.donors, !in_pass, ring_keys);
+			if (!fill && in_pass && !lane)
 				join = !(flags_or(&p.flags[fr.off + i], WIDE) & WIDE);
 		}
 		list_push(join, entry, p.wide_all, &p.nlist[nl_wide_all(p)]);
 		if (!fill && iter + 1 < p.maxiter)
-			mark_dirty(p, changed, f, i, iter + 1);
+			mark_dirty(p, changed && !lane, f, i, iter + 1);
 	}
 }
 
@@ -1227,14 +1387,6 @@ struct BlockShared {
 	int nseed[BLOCK / 64];
 };
 
-/* A 64-bit value that is the same in every lane, as a wave-uniform value. */
-__device__ __forceinline__ unsigned long long
-uniform_u64(unsigned long long v)
-{
-	unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
-	unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
-	return (unsigned long long)hi << 32 | lo;
-}
 
 /* Move this wave's buffered seeds to the global list (whole wave). */
 __device__ __forceinline__ void
