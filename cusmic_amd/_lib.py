@@ -61,7 +61,7 @@ class RaggedOptions(ctypes.Structure):
 
 PROFILE_FIELDS = ("detect_ms", "replace_ms", "prologue_ms", "epilogue_ms", "detect_launches",
                   "detect_pixels", "flagged", "seeds", "candidates", "grown", "widened",
-                  "significance_ms", "detect_pass_pixels", "replace_pass_pixels")
+                  "significance_ms", "detect_pass_pixels", "replace_pass_pixels", "first_pass_ms")
 
 
 class CusmicError(RuntimeError):

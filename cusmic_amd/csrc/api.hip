@@ -251,7 +251,7 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	size_t n_todo = size_t(std::max(maxiter, 1)) * nf;
 	size_t n_queue = size_t(std::max(maxiter, 1));
 	size_t n_nlist = 4 + 9 * size_t(std::max(maxiter, 1));
-	size_t words = n_counts + n_todo + n_queue + n_nlist + 2 + 3 * size_t(nblocks);
+	size_t words = n_counts + n_todo + n_queue + n_nlist + 2 + 3 * size_t(nblocks) + size_t(ntiles);
 	size_t off_frames = 0;
 	size_t off_words = align_up(nf * sizeof(Frame), 256);
 	size_t off_dlist = align_up(off_words + words * 4, 256);
@@ -296,6 +296,7 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	p.block_pass = reinterpret_cast<unsigned int *>(p.status + 2);
 	p.changed_tag = p.block_pass + nblocks;
 	p.dirty_tag = p.changed_tag + nblocks;
+	p.tile_skip = p.dirty_tag + nblocks;
 	p.changed_list = reinterpret_cast<unsigned long long *>(base + off_clist);
 	p.nblocks = nblocks;
 	p.dirty_list = reinterpret_cast<unsigned long long *>(base + off_dlist);
@@ -323,40 +324,54 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 
 	int cus = device_cus();
 	bool model = false; /* some frame uses the noise model instead of an error map */
-	for (size_t f = 0; f < nf; ++f)
+	bool bg_any = !std::isnan(o.background); /* some frame has a background */
+	for (size_t f = 0; f < nf; ++f) {
 		model |= ims[f].error == nullptr;
+		bg_any |= ims[f].background != nullptr;
+	}
 	timer tm(stream, profile != nullptr);
+	tm.mark(-1);
 	hipLaunchKernelGGL(zero_words, dim3(unsigned(std::min<size_t>((words + 255) / 256, 4096))),
 		dim3(256), 0, stream, w32, words);
 	hipLaunchKernelGGL(zero_words, dim3(1), dim3(256), 0, stream,
 		reinterpret_cast<unsigned int *>(p.work), n_queue * 2);
-	tm.mark(-1);
-	hipLaunchKernelGGL(prologue, dim3(nchunks), dim3(BLOCK), 0, stream, p);
+	hipLaunchKernelGGL(zero_groups, dim3(unsigned(std::min<size_t>((total / 16 + 256) / 256, size_t(cus) * 8))),
+		dim3(256), 0, stream, reinterpret_cast<uint4 *>(p.flags), (total + 15) / 16);
+	if (donors)
+		hipLaunchKernelGGL(reset_donors, dim3(unsigned(std::min<size_t>((2 * total + 255) / 256, 8192))),
+			dim3(256), 0, stream, donors, 2 * total);
 	tm.mark(CUSMIC_PROFILE_PROLOGUE_MS);
+	/* the first pass, with the prologue's work, in persistent workgroups: as
+	 * many as fit on the device */
+	using kernel_t = void (*)(Params, int);
+	kernel_t sig;
+	if (model)
+		sig = laplacian ? (bg_any ? significance_model<false, true> : significance_model<false, false>)
+				: (bg_any ? significance_model<true, true> : significance_model<true, false>);
+	else
+		sig = laplacian ? (bg_any ? significance_maps<false, true> : significance_maps<false, false>)
+				: (bg_any ? significance_maps<true, true> : significance_maps<true, false>);
+	int sig_per_cu = 0;
+	hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&sig_per_cu, sig, BLOCK, 0));
+	const int sig_resident = cus * std::max(sig_per_cu, 1);
+	hipLaunchKernelGGL(sig, dim3(unsigned(std::min(nruns, sig_resident))), dim3(BLOCK), 0, stream, p, 0);
+	tm.mark(CUSMIC_PROFILE_FIRST_PASS_MS);
 	unsigned replace_grid = unsigned(cus) * 8;
 	hipLaunchKernelGGL(replace_pass, dim3(replace_grid), dim3(BLOCK), 0, stream, p, 0, 1);
 	hipLaunchKernelGGL(replace_widened, dim3(cus), dim3(BLOCK), 0, stream, p, 0, 1);
 	tm.mark(CUSMIC_PROFILE_REPLACE_MS);
-	/* persistent significance workgroups: as many as fit on the device */
-	auto sig = model ? (laplacian ? significance_model<false> : significance_model<true>)
-			 : (laplacian ? significance_maps<false> : significance_maps<true>);
-	int sig_per_cu = 0;
-	hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&sig_per_cu, sig, BLOCK, 0));
-	const int sig_resident = cus * std::max(sig_per_cu, 1);
 	unsigned list_grid = unsigned(cus) * 4;
 	auto blocks = model ? (laplacian ? significance_blocks<true, false> : significance_blocks<true, true>)
 			    : (laplacian ? significance_blocks<false, false> : significance_blocks<false, true>);
 	int blocks_per_cu = 0;
 	hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, blocks, BLOCK, 0));
+	const unsigned blocks_grid = unsigned(cus * std::max(blocks_per_cu, 1));
 	for (int it = 0; it < maxiter; ++it) {
-		if (it == 0)
-			hipLaunchKernelGGL(sig, dim3(unsigned(std::min(nruns, sig_resident))), dim3(BLOCK), 0,
-				stream, p, it);
-		else {
+		/* pass 0: the blocks of tiles with a hole, after the fill; later
+		 * passes: the blocks near a pixel the last replacement changed */
+		if (it > 0)
 			hipLaunchKernelGGL(expand_dirty, dim3(unsigned(cus) * 2), dim3(BLOCK), 0, stream, p, it);
-			hipLaunchKernelGGL(blocks, dim3(unsigned(cus * std::max(blocks_per_cu, 1))), dim3(BLOCK),
-				0, stream, p, it);
-		}
+		hipLaunchKernelGGL(blocks, dim3(blocks_grid), dim3(BLOCK), 0, stream, p, it);
 		tm.mark(CUSMIC_PROFILE_SIGNIFICANCE_MS);
 		hipLaunchKernelGGL(fine, dim3(list_grid), dim3(BLOCK), 0, stream, p, it);
 		hipLaunchKernelGGL(grow_first, dim3(list_grid), dim3(BLOCK), 0, stream, p, it);
@@ -395,7 +410,8 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 		for (int k = 0; k < CUSMIC_PROFILE_SIZE; ++k)
 			profile[k] = 0;
 		tm.collect(profile);
-		profile[CUSMIC_PROFILE_DETECT_MS] += profile[CUSMIC_PROFILE_SIGNIFICANCE_MS];
+		profile[CUSMIC_PROFILE_DETECT_MS] +=
+			profile[CUSMIC_PROFILE_SIGNIFICANCE_MS] + profile[CUSMIC_PROFILE_FIRST_PASS_MS];
 		profile[CUSMIC_PROFILE_DETECT_LAUNCHES] = maxiter;
 		/* frame f ran detection passes until one found nothing new (or
 		 * maxiter), and a replacement pass after each that found pixels */

@@ -29,10 +29,11 @@ constexpr int RUN_TILES = 8;
  * dirty by the previous replacement. */
 constexpr int BLK = 8;
 
-/* Per-pixel flags (u8). A separate u16 plane holds the stamp of the last pass
- * that computed S' at the pixel: S' is current iff that stamp equals the stamp
- * of the last pass that processed the pixel (the first pass processes every
- * pixel, later passes the dirty blocks, Params::block_pass). */
+/* Per-pixel flags (u8). The first pass marks the pixels where it computed S'
+ * with SCR. A block that a block pass has processed (Params::block_pass
+ * nonzero) keeps in a u16 plane, for each of its pixels, the stamp of the pass
+ * that computed S' there, or 0: S' is current iff that stamp equals the
+ * block's. */
 constexpr uint32_t HOLE = 1;    /* input pixel is nonfinite */
 constexpr uint32_t EXCL = 2;    /* hole or masked: never a seed or donor */
 constexpr uint32_t CRAY = 4;    /* flagged as cosmic ray (the output mask) */
@@ -41,6 +42,7 @@ constexpr uint32_t NEWP = 16;   /* flagged in the latest detection pass */
 constexpr uint32_t REDO = 32;   /* older flagged pixel on this pass's redo list */
 constexpr uint32_t WIDE = 64;   /* on the list of pixels replaced from a
 				 * window wider than 5 x 5 */
+constexpr uint32_t SCR = 128;   /* the first pass computed S' here */
 
 constexpr double FINE_FLOOR = 0.01;   /* src/filters.cuh:21 */
 constexpr double SIGNAL_FLOOR = 1e-5; /* src/filters.cuh:22 */
@@ -66,7 +68,7 @@ struct Frame {
 /* Pointwise passes stream frames in chunks of CHUNK consecutive pixels. */
 constexpr int CHUNK = 8192;
 
-/* Device counters of one call; zeroed by reset_counters. */
+/* Per-frame counters of one call, zeroed at its start. */
 struct Counts {
 	int holes, allowed, donors, error;
 };
@@ -77,7 +79,7 @@ struct Params {
 	double contrast, threshold, neighbor, bg;  /* bg: scalar background or NaN */
 	double *clean;
 	uint8_t *flags;
-	uint16_t *stamp;                 /* S' stamps */
+	uint16_t *stamp;                 /* S' stamps of processed blocks */
 	uint8_t *mask;
 	double *sprime;                  /* S' where the stamp is current */
 	unsigned long long *list_b;      /* seed candidates, then first-growth pixels */
@@ -104,8 +106,8 @@ struct Params {
 	unsigned int *dirty_tag;         /* [nblocks] stamp of the last pass that
 					  * listed the block as dirty */
 	unsigned long long *changed_list; /* frame << 32 | block, per pass */
-	unsigned long long *dirty_list;  /* frame << 32 | block of the blocks a pass
-					  * after the first processes */
+	unsigned long long *dirty_list;  /* frame << 32 | block of the blocks the
+					  * block pass processes */
 	int *status;                     /* [0] error bits, [1] nonzero if a clean
 					  * sample is infinite (D + bg overflowed) */
 	unsigned long long *work;        /* [maxiter] in-frame pixels of processed tiles */
@@ -114,8 +116,11 @@ struct Params {
 	int *run_frame;                  /* [nruns] frame of each first-pass run */
 	int nruns;
 	int nblocks;
-	unsigned int *block_pass;        /* [nblocks] stamp of the last pass after
-					  * the first that processed the block, or 0 */
+	unsigned int *block_pass;        /* [nblocks] stamp of the last block pass
+					  * that processed the block, or 0 */
+	unsigned int *tile_skip;         /* [ntiles] nonzero if the first pass left
+					  * the tile to the block pass after the hole
+					  * fill */
 };
 
 /* Tiles per run, in run order: frame-major, then row-major over blocks of

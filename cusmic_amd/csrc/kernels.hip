@@ -2,8 +2,9 @@
  * Cleaning kernels for gfx950 (MI355X), one wave64 = 64 lanes.
  *
  * Pass structure of one call (all on the caller's stream, no host round trip):
- *   reset_counters -> prologue -> replace(fill holes) ->
- *   maxiter x [detect -> replace] -> epilogue
+ *   zero counters and flags -> first pass (prologue + dense detection) ->
+ *   replace(fill holes) -> maxiter x [block pass + detect -> replace] ->
+ *   epilogue
  * Every per-frame decision of the reference loop (hole fill, early exit, the
  * "no finite replacement donors" error) is taken on the device from counters;
  * the host reads them once after the last pass.
@@ -135,6 +136,24 @@ zero_words(unsigned int *words, size_t n)
 		words[i] = 0;
 }
 
+/* Zero n 16-byte groups (the flags plane). */
+__global__ void
+zero_groups(uint4 *groups, size_t n)
+{
+	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+	     i += (size_t)gridDim.x * blockDim.x)
+		groups[i] = make_uint4(0, 0, 0, 0);
+}
+
+/* Donor records start as (-1, -1): no donor. */
+__global__ void
+reset_donors(int32_t *donors, size_t n)
+{
+	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+	     i += (size_t)gridDim.x * blockDim.x)
+		donors[i] = -1;
+}
+
 /* Iterate the pixels of one tile: lane -> column, 4 rows per pass. */
 struct TilePixels {
 	int x0, y0, w, h;
@@ -171,80 +190,8 @@ set_tables(Params p)
 	}
 }
 
-/* Pointwise passes: one workgroup per chunk of CHUNK consecutive pixels of a
- * frame, in batches of PER pixels per thread whose loads are all issued
- * before the first use. */
-constexpr int PER = 8;
-
-/* Prologue (src/image.cuh:24-48): holes, exclusions, C = (hole ? 0 : D) + bg,
- * per-frame counts of holes, allowed pixels and finite donors, and the list of
- * holes for the fill. */
-__global__ __launch_bounds__(BLOCK) void
-prologue(Params p)
-{
-	__shared__ int red[4];
-	const int f = p.chunk_frame[blockIdx.x];
-	const Frame fr = p.frames[f];
-	const int n = fr.w * fr.h, start = (blockIdx.x - fr.chunk0) * CHUNK + threadIdx.x;
-	const bool bg_map = fr.background != nullptr, bg = bg_map || !isnan(p.bg);
-	int holes = 0, allowed = 0, donors = 0;
-	bool infinite = false;
-	for (int base = start; base < start + CHUNK; base += PER * BLOCK) {
-		double d[PER], b[PER];
-		uint8_t m[PER];
-#pragma unroll
-		for (int k = 0; k < PER; ++k) {
-			int i = base + k * BLOCK;
-			bool in = i < n;
-			d[k] = in ? fr.data[i] : 0.0;
-			b[k] = in && bg_map ? fr.background[i] : p.bg;
-			m[k] = in && fr.mask ? fr.mask[i] : 0;
-		}
-#pragma unroll
-		for (int k = 0; k < PER; ++k) {
-			int i = base + k * BLOCK;
-			bool in = i < n;
-			bool hole = in && !isfinite(d[k]);
-			if (in) {
-				bool allow = !hole && !m[k];
-				double c = hole ? 0.0 : d[k];
-				if (bg)
-					c += 1 * b[k];
-				long long g = fr.off + i;
-				p.clean[g] = c;
-				p.flags[g] = uint8_t((hole ? HOLE : 0u) | (allow ? 0u : EXCL));
-				p.stamp[g] = 0;
-				if (p.donors) {
-					p.donors[2 * g] = -1;
-					p.donors[2 * g + 1] = -1;
-				}
-				holes += hole;
-				allowed += allow;
-				donors += allow && isfinite(c);
-				infinite |= isinf(c);
-			}
-			/* hole list (only consulted when maxiter > 0) */
-			list_push(hole, list_entry(f, i), p.list, &p.nlist[1]);
-		}
-	}
-	/* an infinite clean sample (D + bg overflowed) can make the Laplacian
-	 * NaN, which the detection then has to keep */
-	if (__ballot(infinite) && !(threadIdx.x & 63))
-		atomicOr(&p.status[1], 1);
-	holes = block_sum(holes, red);
-	allowed = block_sum(allowed, red);
-	donors = block_sum(donors, red);
-	if (!threadIdx.x) {
-		if (holes)
-			atomicAdd(&p.counts[f].holes, holes);
-		if (allowed)
-			atomicAdd(&p.counts[f].allowed, allowed);
-		if (donors)
-			atomicAdd(&p.counts[f].donors, donors);
-	}
-}
-
-/* maxiter = 0 (src/image.cuh:72-92): background arithmetic only. */
+/* maxiter = 0 (src/image.cuh:72-92): background arithmetic only, one
+ * workgroup per chunk of CHUNK consecutive pixels of a frame. */
 __global__ __launch_bounds__(BLOCK) void
 copy_without_detection(Params p)
 {
@@ -960,14 +907,16 @@ redo_scan(Params p, int iter)
 /* Detection (src/filters.cuh:86-193, src/cleaner.cuh:22-37,102-126)         */
 /*
  * One detection pass is four launches:
- *   significance  dense: S = L / (2 N) over each tile and a 2-pixel halo, kept
- *                 in an LDS ring of rows; S' = S - med5(S) only where
- *                 S > min(t, t_n) (S' <= S because S >= 0); S' is written to a
- *                 sparse plane stamped with the pass; seeds (S' > t) go to a
- *                 list. The first pass walks runs of RUN_TILES tiles down a
- *                 tile column, so rows shared by two tiles are computed once;
- *                 later passes process the BLK x BLK blocks marked dirty, one
- *                 wave per block.
+ *   significance  S = L / (2 N) over a region and a 2-pixel halo in LDS;
+ *                 S' = S - med5(S) only where S > min(t, t_n) (S' <= S because
+ *                 S >= 0), written to a sparse plane; seeds (S' > t) go to a
+ *                 list. The first pass is dense: it walks runs of RUN_TILES
+ *                 tiles down a tile column, so rows shared by two tiles are
+ *                 computed once, and it makes C from the input as it goes (the
+ *                 prologue). The block pass processes the BLK x BLK blocks
+ *                 marked dirty, one wave per block: in pass 0 those of the
+ *                 tiles whose S waits for the hole fill, later those near a
+ *                 pixel the last replacement changed.
  *   fine          per seed: F = max((m3 - med7(m3)) / N, 0.01), m3 = med3(C);
  *                 S' / F > contrast puts the seed on the candidate list
  *   grow_first    per candidate: neighbours with S' > t join the growth list
@@ -1073,16 +1022,75 @@ map_index(int x, int y, int w, int h, int mode)
 	return y * w + x;
 }
 
+/* C at index i of a frame, made from the input as the prologue of
+ * src/image.cuh:24-48 makes it: C = (hole ? 0 : D) + bg, where a nonfinite
+ * data value is a hole (its C changes again in the hole fill). */
+template <bool BG>
+__device__ __forceinline__ double
+input_c(double d, double b, bool has_bg, bool &hole)
+{
+	const bool hl = !isfinite(d);
+	hole |= hl;
+	double c = hl ? 0.0 : d;
+	if (BG && has_bg)
+		c += 1 * b;
+	return c;
+}
+
+template <bool BG>
+__device__ __forceinline__ double
+input_at(const Frame &fr, double bg, bool has_bg, long long i, bool &hole)
+{
+	return input_c<BG>(fr.data[i], BG && fr.background ? fr.background[i] : bg, has_bg, hole);
+}
+
+/* input_at under the Laplacian's border rule (lap_sample). */
+template <bool BG>
+__device__ __forceinline__ double
+lap_input(const Frame &fr, double bg, bool has_bg, int x, int y, int mode, bool &hole)
+{
+	const int w = fr.w, h = fr.h;
+	if (!(x >= 0 && x < w && y >= 0 && y < h)) {
+		if (mode == CUSMIC_CONSTANT)
+			return 0.0;
+		if (mode == CUSMIC_WRAP) {
+			x = border_index(x, w, mode);
+			y = border_index(y, h, mode);
+		} else {
+			x = min(max(x, 0), w - 1);
+			y = min(max(y, 0), h - 1);
+		}
+	}
+	return input_at<BG>(fr, bg, has_bg, (long long)y * w + x, hole);
+}
+
+/* input_at under the median's border rule (med_sample). */
+template <bool BG>
+__device__ __forceinline__ double
+med_input(const Frame &fr, double bg, bool has_bg, int x, int y, int mode, bool &hole)
+{
+	const int bx = border_index(x, fr.w, mode), by = border_index(y, fr.h, mode);
+	return bx < 0 || by < 0 ? 0.0 : input_at<BG>(fr, bg, has_bg, (long long)by * fr.w + bx, hole);
+}
+
 constexpr int SEED_BUF = 512; /* seeds buffered per workgroup between flushes */
 
 struct SigShared {
 	double s[RING][SW];              /* S, row r of a run in slot r % RING */
 	unsigned long long screen[RING]; /* per row: tile columns with S > min(t, t_n) */
-	int bad[RING];                   /* per row: an invalid noise value */
+	int bad[RING];                   /* per row: bit 0 an invalid noise value,
+					  * bit 1 a hole among the samples of C read */
 	uint16_t alist[TILE_W * TILE_H]; /* screened pixels of the tile: row << 6 | lane */
 	unsigned long long seeds[SEED_BUF];
 	int na, nseed, item;
 	unsigned flush_base;
+	int own[3];                      /* the run's holes, allowed pixels, donors */
+};
+
+/* This lane's counts of the pixels of a run it owns. */
+struct OwnCounts {
+	int holes, allowed, donors;
+	bool infinite;
 };
 
 /* Move the buffered seeds of this workgroup to the global list. */
@@ -1101,53 +1109,94 @@ flush_seeds(SigShared &sh, unsigned long long *list, unsigned int *count)
 		sh.nseed = 0;
 }
 
-/* Store one row of S (this lane's slot) and its screen and noise flags. */
+/* Store one row of S (this lane's slot) and its screen, noise and hole flags. */
 __device__ __forceinline__ void
-emit_row(SigShared &sh, int &slot, int lane, double t_min, unsigned long long cols, double v, bool bad)
+emit_row(SigShared &sh, int &slot, int lane, double t_min, unsigned long long cols, double v, bool bad,
+	bool hole)
 {
 	sh.s[slot][lane] = v;
 	unsigned long long m = __ballot(v > t_min) & cols;
-	bool any_bad = __ballot(bad) != 0;
+	bool any_bad = __ballot(bad) != 0, any_hole = __ballot(hole) != 0;
 	if (!lane) {
 		sh.screen[slot] = m;
-		sh.bad[slot] = any_bad;
+		sh.bad[slot] = int(any_bad) | int(any_hole) << 1;
 	}
 	slot = slot + 1 == RING ? 0 : slot + 1;
 }
 
-/* One row of S under the noise model, whose 5x5 median of C is med. */
-template <bool EXACT, bool TAME>
+/* The prologue's work at a row of owned pixels (own: this lane's pixel i of
+ * frame f belongs to the run): C (unless the caller stores it later), the
+ * counts of allowed pixels and finite donors, and EXCL at masked pixels (the
+ * flags plane starts zeroed). A hole makes its tile skip, where the holes are
+ * flagged, listed and counted. */
+template <bool BG>
+__device__ __forceinline__ void
+own_row(const Params &p, const Frame &fr, bool own, long long i, double c, bool hole, bool store_c,
+	OwnCounts &acc)
+{
+	if (own) {
+		const uint8_t m = fr.mask ? fr.mask[i] : 0;
+		const bool allow = !hole && !m;
+		if (store_c)
+			p.clean[fr.off + i] = c;
+		if (m)
+			p.flags[fr.off + i] = uint8_t(EXCL);
+		acc.allowed += allow;
+		if (BG) {
+			acc.donors += allow && isfinite(c);
+			acc.infinite |= isinf(c);
+		}
+	}
+}
+
+/* One row of S under the noise model, whose 5x5 median of C is med; hm: a
+ * hole among the median's samples. */
+template <bool EXACT, bool BG>
 __device__ __forceinline__ void
 model_row(SigShared &sh, int &slot, int lane, double t_min, unsigned long long cols, const Params &p,
-	const Frame &fr, const double *clean, int gx, int gy, double med)
+	const Frame &fr, int f, int gx, int gy, double med, bool hm, bool own, OwnCounts &acc)
 {
 	const int mode = p.mode, w = fr.w, h = fr.h;
-	double qc = lap_sample(clean, gx, gy, w, h, mode) * 0.25;
-	double qu = lap_sample(clean, gx, gy - 1, w, h, mode) * 0.25;
-	double qd = lap_sample(clean, gx, gy + 1, w, h, mode) * 0.25;
-	double ql = lap_sample(clean, gx - 1, gy, w, h, mode) * 0.25;
-	double qr = lap_sample(clean, gx + 1, gy, w, h, mode) * 0.25;
+	const bool has_bg = BG && (fr.background || !isnan(p.bg));
+	bool hc = false, hn = hm;
+	const double cc = lap_input<BG>(fr, p.bg, has_bg, gx, gy, mode, hc);
+	const double qu = lap_input<BG>(fr, p.bg, has_bg, gx, gy - 1, mode, hn) * 0.25;
+	const double qd = lap_input<BG>(fr, p.bg, has_bg, gx, gy + 1, mode, hn) * 0.25;
+	const double ql = lap_input<BG>(fr, p.bg, has_bg, gx - 1, gy, mode, hn) * 0.25;
+	const double qr = lap_input<BG>(fr, p.bg, has_bg, gx + 1, gy, mode, hn) * 0.25;
 	int mi = map_index(gx, gy, w, h, mode);
 	double gg = fr.gain ? fr.gain[mi] : fr.g;
 	double rn = fr.readnoise ? fr.readnoise[mi] : fr.rn;
 	double nz = noise_model(med, gg, rn);
 	bool in = gx >= 0 && gx < w && gy >= 0 && gy < h;
-	emit_row(sh, slot, lane, t_min, cols, laplacian_q<EXACT, TAME>(qc, qu, qd, ql, qr, w == 1) / (2 * nz),
-		(in || mode == CUSMIC_WRAP) && invalid_noise(nz));
+	own_row<BG>(p, fr, own, (long long)gy * w + gx, cc, hc, true, acc);
+	emit_row(sh, slot, lane, t_min, cols,
+		laplacian_q<EXACT, !BG>(cc * 0.25, qu, qd, ql, qr, w == 1) / (2 * nz),
+		(in || mode == CUSMIC_WRAP) && invalid_noise(nz), hc || hn);
 }
 
 /* One wave: S of run rows [r0, r0 + NR) into the ring (run row r is frame row
  * yb + r, lane l is column X0 - HS + l), with each row's screened columns
- * (among cols) and whether it holds an invalid noise value. Slots outside the
- * frame are computed from in-bounds samples; the caller overwrites the ones
- * the medians read, unless the mode is WRAP, under which every stencil
- * commutes with the periodic extension. NR is a constant so that the row loop
- * unrolls around its ballots and the loads of several rows are in flight. */
-template <bool MODEL, bool EXACT, bool TAME, int NR>
-__device__ __forceinline__ void
-significance_rows(SigShared &sh, const Params &p, const Frame &fr, int X0, int yb, int r0,
-	double t_min, unsigned long long cols)
+ * (among cols), whether it holds an invalid noise value and whether a sample
+ * of C it read is a hole. C is made from the input here; the rows
+ * [own0, own1) and the columns of cols belong to this run (own_row). The
+ * interior path leaves their C in cst and returns true: a store issued
+ * between the loads would hold up every later load's wait (loads and stores
+ * share one counter), so the caller stores the rows after the step's loads.
+ * The other paths store C at once. Without a background no sample of C is infinite,
+ * so the Laplacian clips with one max instruction; with one the reference's
+ * NaN-keeping clip is used. Slots outside the frame are computed from
+ * in-bounds samples; the caller overwrites the ones the medians read, unless
+ * the mode is WRAP, under which every stencil commutes with the periodic
+ * extension. NR is a constant so that the row loop unrolls around its ballots
+ * and the loads of several rows are in flight. */
+template <bool MODEL, bool EXACT, bool BG, int NR>
+__device__ __forceinline__ bool
+significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0, int yb, int r0,
+	double t_min, unsigned long long cols, int own0, int own1, OwnCounts &acc,
+	double (&cst)[STEP_ROWS / 4])
 {
+	constexpr bool TAME = !BG;
 	/* an opaque lane keeps the compiler from hoisting this step's per-lane
 	 * addresses out of the step loop, where they would stay live across the
 	 * medians */
@@ -1155,69 +1204,113 @@ significance_rows(SigShared &sh, const Params &p, const Frame &fr, int X0, int y
 	asm volatile("" : "+v"(lane));
 	const int mode = p.mode, w = fr.w, h = fr.h, nr = NR;
 	const bool wrap = mode == CUSMIC_WRAP, narrow = w == 1;
-	const double *clean = p.clean + fr.off;
+	const bool has_bg = BG && (fr.background || !isnan(p.bg));
+	const bool own_lane = (cols >> lane) & 1;
 	const int gx = X0 - HS + lane, ya = yb + r0;
 	int slot = r0 % RING;
 	/* the rows and their one-pixel ring lie inside the frame */
 	const bool inner = X0 - HS - 1 >= 0 && X0 - HS + SW < w && ya - 1 >= 0 && ya + nr < h;
 	if ((!MODEL || fr.error) && inner) {
-		const double *rp = clean + (long long)ya * w + gx;
-		const double *ep = fr.error + (long long)ya * w + gx;
-		double qu = rp[-w] * 0.25, qc = rp[0] * 0.25;
-		/* convergent ballots allow only an unroll factor that divides NR */
-		constexpr int UNROLL = NR % 4 == 0 ? 4 : NR % 3 == 0 ? 3 : 1;
-#pragma unroll UNROLL
+		long long o = (long long)ya * w + gx;
+		const double *dp = fr.data + o, *ep = fr.error + o;
+		const double *bp = BG && fr.background ? fr.background + o : nullptr;
+		bool hu = false, hc = false;
+		double qu = input_c<BG>(dp[-w], bp ? bp[-w] : p.bg, has_bg, hu) * 0.25;
+		double cc = input_c<BG>(dp[0], bp ? bp[0] : p.bg, has_bg, hc);
+		double qc = cc * 0.25;
+		/* fully unrolled (constant NR, convergent ballots), so that the rows'
+		 * C stays in registers for the caller to store after the loads */
+#pragma unroll
 		for (int r = 0; r < nr; ++r) {
-			double dn = rp[w], l = rp[-1], rt = rp[1], nz = *ep;
-			double qd = dn * 0.25;
-			emit_row(sh, slot, lane, t_min, cols, laplacian_q<EXACT, TAME>(qc, qu, qd, l * 0.25, rt * 0.25, false) / (2 * nz),
-				invalid_noise(nz));
+			const double dn = dp[w], dl = dp[-1], dr = dp[1], nz = *ep;
+			double bn = p.bg, bl = p.bg, br = p.bg;
+			if (BG && bp) {
+				bn = bp[w];
+				bl = bp[-1];
+				br = bp[1];
+			}
+			bool hd = false, hs = false;
+			const double cd = input_c<BG>(dn, bn, has_bg, hd);
+			const double ql = input_c<BG>(dl, bl, has_bg, hs) * 0.25;
+			const double qr = input_c<BG>(dr, br, has_bg, hs) * 0.25;
+			const double qd = cd * 0.25;
+			const int gy = ya + r;
+			own_row<BG>(p, fr, own_lane && gy >= own0 && gy < own1, o, cc, hc, false, acc);
+			cst[r] = cc;
+			emit_row(sh, slot, lane, t_min, cols, laplacian_q<EXACT, TAME>(qc, qu, qd, ql, qr, false) / (2 * nz),
+				invalid_noise(nz), hu || hc || hd || hs);
 			qu = qc;
 			qc = qd;
-			rp += w;
+			cc = cd;
+			hu = hc;
+			hc = hd;
+			dp += w;
 			ep += w;
+			o += w;
+			if (BG && bp)
+				bp += w;
 		}
+		return true;
 	} else if (!MODEL || fr.error) {
-		double qu = lap_sample(clean, gx, ya - 1, w, h, mode) * 0.25;
-		double qc = lap_sample(clean, gx, ya, w, h, mode) * 0.25;
+		bool hu = false, hc = false;
+		double qu = lap_input<BG>(fr, p.bg, has_bg, gx, ya - 1, mode, hu) * 0.25;
+		double cc = lap_input<BG>(fr, p.bg, has_bg, gx, ya, mode, hc);
 #pragma unroll 1
 		for (int r = 0; r < nr; ++r) {
-			int gy = ya + r;
-			double qd = lap_sample(clean, gx, gy + 1, w, h, mode) * 0.25;
-			double ql = lap_sample(clean, gx - 1, gy, w, h, mode) * 0.25;
-			double qr = lap_sample(clean, gx + 1, gy, w, h, mode) * 0.25;
-			double nz = fr.error[map_index(gx, gy, w, h, mode)];
-			bool in = gx >= 0 && gx < w && gy >= 0 && gy < h;
-			emit_row(sh, slot, lane, t_min, cols, laplacian_q<EXACT, TAME>(qc, qu, qd, ql, qr, narrow) / (2 * nz),
-				(in || wrap) && invalid_noise(nz));
-			qu = qc;
-			qc = qd;
+			const int gy = ya + r;
+			bool hd = false, hs = false;
+			const double cd = lap_input<BG>(fr, p.bg, has_bg, gx, gy + 1, mode, hd);
+			const double ql = lap_input<BG>(fr, p.bg, has_bg, gx - 1, gy, mode, hs) * 0.25;
+			const double qr = lap_input<BG>(fr, p.bg, has_bg, gx + 1, gy, mode, hs) * 0.25;
+			const double nz = fr.error[map_index(gx, gy, w, h, mode)];
+			const bool in = gx >= 0 && gx < w && gy >= 0 && gy < h;
+			own_row<BG>(p, fr, own_lane && gy >= own0 && gy < own1, (long long)gy * w + gx, cc, hc, true,
+				acc);
+			emit_row(sh, slot, lane, t_min, cols,
+				laplacian_q<EXACT, TAME>(cc * 0.25, qu, cd * 0.25, ql, qr, narrow) / (2 * nz),
+				(in || wrap) && invalid_noise(nz), hu || hc || hd || hs);
+			qu = cc * 0.25;
+			cc = cd;
+			hu = hc;
+			hc = hd;
 		}
 	} else {
 		/* noise model: N from strip-shared 5x5 medians, four rows at a time */
 #pragma unroll 1
 		for (int g = 0; g < nr; g += 4) {
-			int gy0 = ya + g;
+			const int gy0 = ya + g;
 			double rows[8 * 5], med[4];
+			bool hm = false;
 #pragma unroll
 			for (int j = 0; j < 8; ++j)
 #pragma unroll
 				for (int c = 0; c < 5; ++c)
-					rows[j * 5 + c] = med_sample(clean, gx - 2 + c, gy0 - 2 + j, w, h, mode);
+					rows[j * 5 + c] = med_input<BG>(fr, p.bg, has_bg, gx - 2 + c, gy0 - 2 + j, mode, hm);
 			strip_median5x4(rows, med);
 			const int n_out = min(4, nr - g);
-			model_row<EXACT, TAME>(sh, slot, lane, t_min, cols, p, fr, clean, gx, gy0, med[0]);
+#define MODEL_ROW(k)                                                                              \
+	model_row<EXACT, BG>(sh, slot, lane, t_min, cols, p, fr, f, gx, gy0 + (k), med[k], hm,       \
+		own_lane && gy0 + (k) >= own0 && gy0 + (k) < own1, acc)
+			MODEL_ROW(0);
 			if (n_out > 1)
-				model_row<EXACT, TAME>(sh, slot, lane, t_min, cols, p, fr, clean, gx, gy0 + 1, med[1]);
+				MODEL_ROW(1);
 			if (n_out > 2)
-				model_row<EXACT, TAME>(sh, slot, lane, t_min, cols, p, fr, clean, gx, gy0 + 2, med[2]);
+				MODEL_ROW(2);
 			if (n_out > 3)
-				model_row<EXACT, TAME>(sh, slot, lane, t_min, cols, p, fr, clean, gx, gy0 + 3, med[3]);
+				MODEL_ROW(3);
+#undef MODEL_ROW
 		}
 	}
+	return false;
 }
 
-template <bool MODEL, bool EXACT, bool TAME>
+/* The first detection pass, which also does the prologue's work (C, flags,
+ * counts, hole list) for the pixels of each run. A tile whose samples of C
+ * hold a hole gets no S' here: its S depends on the hole fill, which runs
+ * after this pass, so its blocks go to the dirty list of the block pass that
+ * follows the fill (tile_skip marks it). Elsewhere S' is exact, and the
+ * screened pixels carry SCR. */
+template <bool MODEL, bool EXACT, bool BG>
 __device__ __forceinline__ void
 significance_pass(SigShared &sh, const Params &p, int iter)
 {
@@ -1225,7 +1318,7 @@ significance_pass(SigShared &sh, const Params &p, int iter)
 	const int mode = p.mode;
 	const bool wrap = mode == CUSMIC_WRAP;
 	const double t_cr = p.threshold, t_min = fmin(p.threshold, p.neighbor);
-	const unsigned stamp = pass_stamp(iter);
+	const unsigned dtag = pass_stamp(iter);
 	const unsigned limit = (unsigned)p.nruns;
 	unsigned int *seeds = &p.nlist[nl_seed(p, iter)];
 	unsigned next = 0;
@@ -1233,6 +1326,7 @@ significance_pass(SigShared &sh, const Params &p, int iter)
 	if (!threadIdx.x) {
 		next = atomicAdd(&p.queue[iter], 1u);
 		sh.nseed = 0;
+		sh.own[0] = sh.own[1] = sh.own[2] = 0;
 	}
 
 	for (;;) {
@@ -1253,8 +1347,9 @@ significance_pass(SigShared &sh, const Params &p, int iter)
 		const int lr = item - fr.run0, tx = lr % fr.tiles_x, ty0 = lr / fr.tiles_x * RUN_TILES;
 		const int K = min(RUN_TILES, fr.tiles_y - ty0);
 		const int w = fr.w, h = fr.h, X0 = tx * TILE_W, Y0 = ty0 * TILE_H, yb = Y0 - HS;
-		const int ncols = min(TILE_W, w - X0);
+		const int ncols = min(TILE_W, w - X0), own1 = min(Y0 + TILE_H * K, h);
 		const unsigned long long cols = ((1ull << ncols) - 1) << HS;
+		OwnCounts acc{0, 0, 0, false};
 
 		for (int s = 0; s < K; ++s) {
 			/* per-step opaque thread index, as in significance_rows */
@@ -1263,17 +1358,25 @@ significance_pass(SigShared &sh, const Params &p, int iter)
 			const int lane = tid & 63;
 			/* 1. S rows: the first tile needs rows [0, STEP_ROWS) of the run,
 			 * each later one TILE_H more */
+			double cst[STEP_ROWS / 4];
+			bool deferred;
+			int r0, nr;
 			if (s == 0) {
-				significance_rows<MODEL, EXACT, TAME, STEP_ROWS / 4>(sh, p, fr, X0, yb,
-					wave * (STEP_ROWS / 4), t_min, cols);
+				r0 = wave * (STEP_ROWS / 4);
+				nr = STEP_ROWS / 4;
+				deferred = significance_rows<MODEL, EXACT, BG, STEP_ROWS / 4>(sh, p, fr, f, X0, yb,
+					r0, t_min, cols, Y0, own1, acc, cst);
 			} else {
 				__syncthreads(); /* the last tile's medians are done with the ring */
-				significance_rows<MODEL, EXACT, TAME, TILE_H / 4>(sh, p, fr, X0, yb,
-					TILE_H * s + 2 * HS + wave * (TILE_H / 4), t_min, cols);
+				r0 = TILE_H * s + 2 * HS + wave * (TILE_H / 4);
+				nr = TILE_H / 4;
+				deferred = significance_rows<MODEL, EXACT, BG, TILE_H / 4>(sh, p, fr, f, X0, yb,
+					r0, t_min, cols, Y0, own1, acc, cst);
 			}
 			const int ty = Y0 + TILE_H * s; /* first frame row of the tile */
+			const int nrows = min(TILE_H, h - ty);
 			if (!tid) {
-				work += (unsigned long long)(ncols * min(TILE_H, h - ty));
+				work += (unsigned long long)(ncols * nrows);
 				sh.na = 0;
 			}
 			__syncthreads();
@@ -1297,26 +1400,64 @@ significance_pass(SigShared &sh, const Params &p, int iter)
 
 			/* 3. screened pixels of the tile: S' <= S, so only S > min(t, t_n)
 			 * can pass a threshold; with an invalid noise value among the rows
-			 * the medians read, every pixel is kept */
-			const bool unsafe = __ballot(lane < STEP_ROWS && sh.bad[(r_lo + lane) % RING]) != 0;
-			for (int j = 0; j < TILE_H / 4; ++j) {
-				int row = wave * (TILE_H / 4) + j;
-				if (ty + row >= h)
-					break;
-				unsigned long long m = unsafe ? cols : sh.screen[(r_lo + HS + row) % RING];
-				lds_append(((m >> lane) & 1) != 0, (uint16_t)((row << 6) | lane), sh.alist, &sh.na);
+			 * the medians read, every pixel is kept. With a hole among the
+			 * samples, the tile's blocks are left to the block pass after the
+			 * fill. */
+			const int rb = lane < STEP_ROWS ? sh.bad[(r_lo + lane) % RING] : 0;
+			const bool unsafe = __ballot(rb & 1) != 0, skip = __ballot(rb & 2) != 0;
+			if (skip) {
+				const int bx0 = X0 / BLK, by0 = ty / BLK;
+				const int nbx = (X0 + ncols - 1) / BLK - bx0 + 1;
+				const int nb = nbx * ((ty + nrows - 1) / BLK - by0 + 1); /* at most 9 x 4 */
+				bool fresh = false;
+				int q = 0;
+				if (tid < nb) {
+					q = fr.block0 + (by0 + tid / nbx) * fr.blocks_x + bx0 + tid % nbx;
+					fresh = __atomic_load_n(&p.dirty_tag[q], __ATOMIC_RELAXED) != dtag &&
+						atomicExch(&p.dirty_tag[q], dtag) != dtag;
+				}
+				list_push(fresh, list_entry(f, q), p.dirty_list, &p.nlist[nl_dirty(p, iter)]);
+				if (!tid)
+					p.tile_skip[fr.tile0 + (ty0 + s) * fr.tiles_x + tx] = 1;
+				/* the tile's holes (only a skipped tile has any): the fill's
+				 * list and count. The trip count is uniform per wave. */
+				for (int k = tid; k < TILE_W * TILE_H; k += BLOCK) {
+					const int row = k / TILE_W, col = k % TILE_W, i = (ty + row) * w + X0 + col;
+					const bool hole = row < nrows && col < ncols && !isfinite(fr.data[i]);
+					if (hole)
+						p.flags[fr.off + i] = uint8_t(HOLE | EXCL);
+					acc.holes += hole;
+					list_push(hole, list_entry(f, i), p.list, &p.nlist[1]);
+				}
+			} else {
+				for (int j = 0; j < TILE_H / 4; ++j) {
+					int row = wave * (TILE_H / 4) + j;
+					if (row >= nrows)
+						break;
+					unsigned long long m = unsafe ? cols : sh.screen[(r_lo + HS + row) % RING];
+					lds_append(((m >> lane) & 1) != 0, (uint16_t)((row << 6) | lane), sh.alist, &sh.na);
+				}
 			}
 			__syncthreads();
 
-			/* 4. S' at the screened pixels, stamped into the sparse plane; those
-			 * above t are buffered as seeds. Holes (S' = 0) and excluded pixels
+			/* the deferred C of this wave's rows: its latency overlaps the
+			 * medians below */
+			if (deferred) {
+				const int ya = yb + r0;
+				double *cp = p.clean + fr.off + (long long)ya * w + X0 - HS + lane;
+				const bool own_lane = (cols >> lane) & 1;
+#pragma unroll
+				for (int r = 0; r < STEP_ROWS / 4; ++r)
+					if (r < nr && own_lane && ya + r >= Y0 && ya + r < own1)
+						cp[(long long)r * w] = cst[r];
+			}
+
+			/* 4. S' at the screened pixels, which get SCR (no hole lies in the
+			 * tile); those above t are buffered as seeds (a batch adds at most
+			 * BLOCK, and the buffer keeps room for that many). Excluded pixels
 			 * are dropped by the consumers, which read the flags anyway. */
 			const int na = sh.na;
 			for (int base = 0; base < na; base += BLOCK) {
-				/* a batch adds at most BLOCK seeds */
-				__syncthreads();
-				if (sh.nseed + BLOCK > SEED_BUF)
-					flush_seeds(sh, p.list_b, seeds);
 				int idx = base + tid;
 				bool seed = false;
 				int i = 0;
@@ -1335,11 +1476,43 @@ significance_pass(SigShared &sh, const Params &p, int iter)
 					median25(v, &m);
 					double sp = v[12] - m;
 					p.sprime[fr.off + i] = sp;
-					p.stamp[fr.off + i] = (uint16_t)stamp;
+					p.flags[fr.off + i] = uint8_t(SCR | (fr.mask && fr.mask[i] ? EXCL : 0u));
 					seed = sp > t_cr;
 				}
 				lds_append(seed, list_entry(f, i), sh.seeds, &sh.nseed);
+				__syncthreads();
+				if (sh.nseed + BLOCK > SEED_BUF)
+					flush_seeds(sh, p.list_b, seeds);
 			}
+		}
+		/* the run's counts: one LDS atomic per wave, one global atomic per
+		 * count and run */
+		{
+			int c3[3] = {acc.holes, acc.allowed, BG ? acc.donors : acc.allowed};
+#pragma unroll
+			for (int k = 0; k < 3; ++k) {
+				for (int o = 32; o; o >>= 1)
+					c3[k] += __shfl_xor(c3[k], o, 64);
+			}
+			const bool inf = BG && __ballot(acc.infinite) != 0;
+			if (!(threadIdx.x & 63)) {
+#pragma unroll
+				for (int k = 0; k < 3; ++k)
+					if (c3[k])
+						atomicAdd(&sh.own[k], c3[k]);
+				if (inf)
+					atomicOr(&p.status[1], 1);
+			}
+		}
+		__syncthreads();
+		if (!threadIdx.x) {
+			if (sh.own[0])
+				atomicAdd(&p.counts[f].holes, sh.own[0]);
+			if (sh.own[1])
+				atomicAdd(&p.counts[f].allowed, sh.own[1]);
+			if (sh.own[2])
+				atomicAdd(&p.counts[f].donors, sh.own[2]);
+			sh.own[0] = sh.own[1] = sh.own[2] = 0;
 		}
 	}
 	flush_seeds(sh, p.list_b, seeds);
@@ -1405,11 +1578,14 @@ flush_wave_seeds(unsigned long long *buf, int *n, unsigned long long *list, unsi
 	__builtin_amdgcn_wave_barrier();
 }
 
-/* Passes after the first: one wave per dirty block, independent of the other
- * waves. S over the block and a 2-pixel halo goes to the wave's LDS region;
- * a slot outside the frame holds S at its mapped in-frame pixel (median border
- * mode; 0 for CONSTANT), which WRAP's periodic stencils also give. Screened
- * pixels of the block get S' as in the first pass. */
+/* Block pass: one wave per dirty block, independent of the other waves. S
+ * over the block and a 2-pixel halo goes to the wave's LDS region; a slot
+ * outside the frame holds S at its mapped in-frame pixel (median border mode;
+ * 0 for CONSTANT), which WRAP's periodic stencils also give. Screened pixels
+ * of the block get S' as in the first pass, and every pixel of the block the
+ * pass's stamp (screened) or 0. In pass 0 the blocks are those of the tiles
+ * that the first pass left to this pass after the hole fill: only pixels of
+ * those tiles become seeds (the first pass has the others). */
 template <bool MODEL, bool EXACT, bool TAME>
 __device__ __forceinline__ void
 significance_blocks_pass(BlockShared &sh, const Params &p, int iter)
@@ -1441,7 +1617,8 @@ significance_blocks_pass(BlockShared &sh, const Params &p, int iter)
 		const int X0 = lb % fr.blocks_x * BLK, Y0 = lb / fr.blocks_x * BLK;
 		if (!lane) {
 			p.block_pass[b] = stamp;
-			work += (unsigned long long)(min(BLK, w - X0) * min(BLK, h - Y0));
+			if (iter)
+				work += (unsigned long long)(min(BLK, w - X0) * min(BLK, h - Y0));
 		}
 		bool bad = false;
 		/* the region and its one-pixel ring lie inside the frame */
@@ -1482,17 +1659,22 @@ significance_blocks_pass(BlockShared &sh, const Params &p, int iter)
 		const int ox = lane % BLK, oy = lane / BLK, x = X0 + ox, y = Y0 + oy;
 		const int c = (oy + HS) * BR + ox + HS;
 		bool seed = false;
-		if (x < w && y < h && (unsafe || S[c] > t_min)) {
-			double v[25], m;
+		if (x < w && y < h) {
+			const long long g = fr.off + (long long)y * w + x;
+			const bool screened = unsafe || S[c] > t_min;
+			if (screened) {
+				double v[25], m;
 #pragma unroll
-			for (int k = 0; k < 25; ++k)
-				v[k] = S[c + (k / 5 - 2) * BR + k % 5 - 2];
-			median25(v, &m);
-			double sp = v[12] - m;
-			long long g = fr.off + (long long)y * w + x;
-			p.sprime[g] = sp;
-			p.stamp[g] = (uint16_t)stamp;
-			seed = sp > t_cr;
+				for (int k = 0; k < 25; ++k)
+					v[k] = S[c + (k / 5 - 2) * BR + k % 5 - 2];
+				median25(v, &m);
+				double sp = v[12] - m;
+				p.sprime[g] = sp;
+				seed = sp > t_cr;
+			}
+			p.stamp[g] = uint16_t(screened ? stamp : 0u);
+			if (!iter && seed)
+				seed = p.tile_skip[fr.tile0 + (y / TILE_H) * fr.tiles_x + x / TILE_W] != 0;
 		}
 		lds_append(seed, list_entry(f, y * w + x), buf, nseed);
 		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1517,31 +1699,26 @@ significance_blocks(Params p, int iter)
 		significance_blocks_pass<MODEL, EXACT, true>(sh, p, iter);
 }
 
-/* Every frame has an error map. The pass is bound by load latency, so the
- * register budget is capped at four waves per SIMD (128 VGPRs, no spills),
- * one workgroup more per CU than the compiler would choose. */
-template <bool EXACT>
+/* The first pass when every frame has an error map. It is bound by load
+ * latency, so the register budget is capped at four waves per SIMD (128
+ * VGPRs, no spills), one workgroup more per CU than the compiler would choose.
+ * BG: some frame has a background. */
+template <bool EXACT, bool BG>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
 significance_maps(Params p, int iter)
 {
 	__shared__ SigShared sh;
-	if (p.status[1])
-		significance_pass<false, EXACT, false>(sh, p, iter);
-	else
-		significance_pass<false, EXACT, true>(sh, p, iter);
+	significance_pass<false, EXACT, BG>(sh, p, iter);
 }
 
 /* Some frame uses the noise model (5x5 medians of C in the dense part):
  * two waves per SIMD. */
-template <bool EXACT>
+template <bool EXACT, bool BG>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2))) void
 significance_model(Params p, int iter)
 {
 	__shared__ SigShared sh;
-	if (p.status[1])
-		significance_pass<true, EXACT, false>(sh, p, iter);
-	else
-		significance_pass<true, EXACT, true>(sh, p, iter);
+	significance_pass<true, EXACT, BG>(sh, p, iter);
 }
 
 __device__ __forceinline__ double
@@ -1690,17 +1867,17 @@ fine(Params p, int iter)
 }
 
 /* S' at pixel (x, y) of a frame if the last pass that processed the pixel
- * (the first pass, or a later one through its dirty block) computed it there
- * (0 at holes, src/filters.cuh:137-146), else -inf: neither can pass a
- * threshold t >= 0. */
+ * computed it there (0 at holes, src/filters.cuh:137-146), else -inf: neither
+ * can pass a threshold t >= 0. A block that no block pass has processed is the
+ * first pass's (SCR); otherwise the pixel's stamp must be the block's. */
 __device__ __forceinline__ double
 sprime_at(const Params &p, long long off, int block0, int blocks_x, int x, int y, int w)
 {
 	long long g = off + (long long)y * w + x;
 	uint32_t fw = p.flags[g];
-	unsigned st = p.stamp[g];
-	unsigned last = max(pass_stamp(0), p.block_pass[block0 + (y / BLK) * blocks_x + x / BLK]);
-	return st == last && !(fw & HOLE) ? p.sprime[g] : -INFINITY;
+	unsigned last = p.block_pass[block0 + (y / BLK) * blocks_x + x / BLK];
+	bool current = last ? p.stamp[g] == last : (fw & SCR) != 0;
+	return current && !(fw & HOLE) ? p.sprime[g] : -INFINITY;
 }
 
 /* First growth (src/filters.cuh:177-193 at t): every pixel next to a

@@ -33,17 +33,22 @@ struct cusmic_ragged_options {
 };
 
 /* Slots of cusmic_ragged_options.profile. Times are summed over launches in
- * milliseconds. DETECT_MS covers whole detection passes, SIGNIFICANCE_MS only
- * their dense significance launches (DETECT_LAUNCHES of them), which processed
- * the DETECT_PIXELS in-frame pixels of their tiles. SEEDS .. WIDENED are list
- * lengths summed over passes. DETECT_PASS_PIXELS and REPLACE_PASS_PIXELS sum
- * frame pixels times the detection and replacement passes the frame ran. */
+ * milliseconds. PROLOGUE_MS covers the set-up launches; FIRST_PASS_MS the
+ * dense first detection pass, which also makes the working image and the
+ * flags from the input; SIGNIFICANCE_MS the block passes (DETECT_LAUNCHES of
+ * them: one per pass, the first after the hole fill); DETECT_MS whole
+ * detection passes, both of these included. DETECT_PIXELS counts the in-frame
+ * pixels that the first pass and the later block passes processed. SEEDS ..
+ * WIDENED are list lengths summed over passes. DETECT_PASS_PIXELS and
+ * REPLACE_PASS_PIXELS sum frame pixels times the detection and replacement
+ * passes the frame ran. */
 enum cusmic_profile {
 	CUSMIC_PROFILE_DETECT_MS, CUSMIC_PROFILE_REPLACE_MS, CUSMIC_PROFILE_PROLOGUE_MS,
 	CUSMIC_PROFILE_EPILOGUE_MS, CUSMIC_PROFILE_DETECT_LAUNCHES, CUSMIC_PROFILE_DETECT_PIXELS,
 	CUSMIC_PROFILE_FLAGGED, CUSMIC_PROFILE_SEEDS, CUSMIC_PROFILE_CANDIDATES,
 	CUSMIC_PROFILE_GROWN, CUSMIC_PROFILE_WIDENED, CUSMIC_PROFILE_SIGNIFICANCE_MS,
-	CUSMIC_PROFILE_DETECT_PASS_PIXELS, CUSMIC_PROFILE_REPLACE_PASS_PIXELS, CUSMIC_PROFILE_SIZE
+	CUSMIC_PROFILE_DETECT_PASS_PIXELS, CUSMIC_PROFILE_REPLACE_PASS_PIXELS,
+	CUSMIC_PROFILE_FIRST_PASS_MS, CUSMIC_PROFILE_SIZE
 };
 
 /* Ragged batch: 1..65535 contiguous device frames of individual shapes,

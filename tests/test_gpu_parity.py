@@ -142,9 +142,9 @@ def test_error_map_scene(cuda, mode, laplacian):
 
 @pytest.mark.parametrize("mode", MODES)
 def test_noise_model_scene(cuda, mode):
-    d, _, _ = scene((133, 97), 9)
+    d, _, m = scene((133, 97), 9, nan=1e-3, mask_frac=2e-3)
     say("noise-model scene", mode)
-    compare([dict(data=d)], border_mode=mode, frame_gain=[2.0], frame_readnoise=[5.0])
+    compare([dict(data=d, mask=m)], border_mode=mode, frame_gain=[2.0], frame_readnoise=[5.0])
 
 
 def test_maps_and_background(cuda):
