@@ -251,7 +251,7 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	size_t n_todo = size_t(std::max(maxiter, 1)) * nf;
 	size_t n_queue = size_t(std::max(maxiter, 1));
 	size_t n_nlist = 4 + 9 * size_t(std::max(maxiter, 1));
-	size_t words = n_counts + n_todo + n_queue + n_nlist + 2 + 3 * size_t(nblocks) + size_t(ntiles);
+	size_t words = n_counts + n_todo + n_queue + n_nlist + 2 + 4 * size_t(nblocks) + size_t(ntiles);
 	size_t off_frames = 0;
 	size_t off_words = align_up(nf * sizeof(Frame), 256);
 	size_t off_dlist = align_up(off_words + words * 4, 256);
@@ -296,7 +296,8 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	p.block_pass = reinterpret_cast<unsigned int *>(p.status + 2);
 	p.changed_tag = p.block_pass + nblocks;
 	p.dirty_tag = p.changed_tag + nblocks;
-	p.tile_skip = p.dirty_tag + nblocks;
+	p.changed_sides = p.dirty_tag + nblocks;
+	p.tile_skip = p.changed_sides + nblocks;
 	p.changed_list = reinterpret_cast<unsigned long long *>(base + off_clist);
 	p.nblocks = nblocks;
 	p.dirty_list = reinterpret_cast<unsigned long long *>(base + off_dlist);
@@ -329,6 +330,7 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 		model |= ims[f].error == nullptr;
 		bg_any |= ims[f].background != nullptr;
 	}
+	p.reach = model ? REACH : REACH_MAPS;
 	timer tm(stream, profile != nullptr);
 	tm.mark(-1);
 	hipLaunchKernelGGL(zero_words, dim3(unsigned(std::min<size_t>((words + 255) / 256, 4096))),

@@ -48,10 +48,16 @@ constexpr double FINE_FLOOR = 0.01;   /* src/filters.cuh:21 */
 constexpr double SIGNAL_FLOOR = 1e-5; /* src/filters.cuh:22 */
 
 /* Detection reach: a change of the clean image at one pixel can change the
- * detection result only within this Chebyshev radius (Laplacian 1 + median 2
- * of the significance + two growth steps 2, or median 1 + median 3 of the fine
- * structure + 2). */
+ * contribution of a candidate (its own test and its two growth steps) only
+ * within this Chebyshev radius: S' reaches 3 with an error map (Laplacian 1 +
+ * median of S 2) and 4 under the noise model (its N is a 5x5 median of C),
+ * growth adds 2; the fine structure reaches 4 (median 1 + median 3), below
+ * either. */
+constexpr int REACH_MAPS = 5;
 constexpr int REACH = 6;
+
+/* Sides of a block that a change at in-block offset (ox, oy) reaches. */
+constexpr unsigned SIDE_L = 1, SIDE_R = 2, SIDE_U = 4, SIDE_D = 8;
 
 struct Frame {
 	const double *data, *error, *gain, *readnoise, *background;
@@ -76,6 +82,7 @@ struct Counts {
 struct Params {
 	const Frame *frames;
 	int nf, ntiles, maxiter, mode, laplacian;
+	int reach;                       /* REACH_MAPS or REACH */
 	double contrast, threshold, neighbor, bg;  /* bg: scalar background or NaN */
 	double *clean;
 	uint8_t *flags;
@@ -103,6 +110,8 @@ struct Params {
 	unsigned int *changed_tag;       /* [nblocks] stamp of the last pass whose
 					  * preceding replacement changed a pixel of
 					  * the block */
+	unsigned int *changed_sides;     /* [nblocks] SIDE_* bits the block's
+					  * changes reach; reset when expanded */
 	unsigned int *dirty_tag;         /* [nblocks] stamp of the last pass that
 					  * listed the block as dirty */
 	unsigned long long *changed_list; /* frame << 32 | block, per pass */

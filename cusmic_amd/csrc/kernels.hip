@@ -370,25 +370,32 @@ mark_dirty(const Params &p, bool changed, int f, int i, int next)
 	bool fresh = false;
 	if (changed) {
 		const Frame fr = p.frames[f];
-		const int x = i % fr.w, y = i / fr.w;
+		const int x = i % fr.w, y = i / fr.w, ox = x % BLK, oy = y % BLK;
 		b = unsigned(fr.block0 + (y / BLK) * fr.blocks_x + x / BLK);
+		const unsigned sides = (ox < p.reach ? SIDE_L : 0u) | (ox >= BLK - p.reach ? SIDE_R : 0u) |
+				       (oy < p.reach ? SIDE_U : 0u) | (oy >= BLK - p.reach ? SIDE_D : 0u);
+		if ((__atomic_load_n(&p.changed_sides[b], __ATOMIC_RELAXED) & sides) != sides)
+			atomicOr(&p.changed_sides[b], sides);
 		fresh = __atomic_load_n(&p.changed_tag[b], __ATOMIC_RELAXED) != tag &&
 			atomicExch(&p.changed_tag[b], tag) != tag;
 	}
 	list_push(fresh, list_entry(f, (int)b), p.changed_list, &p.nlist[nl_changed(p, next)]);
 }
 
-/* Blocks of pass iter: every block within one block (Chebyshev) of a changed
- * block, since REACH < BLK; under WRAP two blocks, because a partial block at
- * the seam is narrower than REACH. Each joins the dirty list once (tagged with
- * the pass), through a per-workgroup LDS buffer and one atomic per flush. */
+/* Blocks of pass iter: the neighbours of a changed block on the sides its
+ * changes reach (the reach is below BLK, so only adjacent blocks), diagonal
+ * ones when both of their sides are reached; under WRAP every block within
+ * two, because a partial block at the seam is narrower than the reach. Each
+ * joins the dirty list once (tagged with the pass); the sides are reset for
+ * the next pass. */
 __global__ __launch_bounds__(BLOCK) void
 expand_dirty(Params p, int iter)
 {
 	static_assert(REACH < BLK, "a change reaches the adjacent blocks only");
 	__shared__ BlockList<BLOCK * 9> out;
 	const unsigned n = p.nlist[nl_changed(p, iter)], tag = pass_stamp(iter);
-	const int r = p.mode == CUSMIC_WRAP ? 2 : 1, side = 2 * r + 1;
+	const bool wrap = p.mode == CUSMIC_WRAP;
+	const int r = wrap ? 2 : 1, side = 2 * r + 1;
 	unsigned int *count = &p.nlist[nl_dirty(p, iter)];
 	if (!threadIdx.x)
 		out.n = 0;
@@ -405,6 +412,7 @@ expand_dirty(Params p, int iter)
 	for (unsigned base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
 		const unsigned e = base + threadIdx.x;
 		int f = 0, bx = 0, by = 0, nbx = 1, nby = 1, block0 = 0;
+		unsigned sides = 0;
 		if (e < n) {
 			unsigned long long entry = p.changed_list[e];
 			f = (int)(entry >> 32);
@@ -412,22 +420,27 @@ expand_dirty(Params p, int iter)
 			block0 = fr.block0;
 			nbx = fr.blocks_x;
 			nby = (fr.h + BLK - 1) / BLK;
-			int lb = (int)(entry & 0xffffffffu) - block0;
+			const int b = (int)(entry & 0xffffffffu), lb = b - block0;
 			bx = lb % nbx;
 			by = lb / nbx;
+			sides = p.changed_sides[b];
+			p.changed_sides[b] = 0;
 		}
 		/* rounds of at most nine neighbours, one buffer flush each */
 		for (int k0 = 0; k0 < side * side; k0 += 9) {
 			__syncthreads();
 			for (int k = k0; k < k0 + 9 && k < side * side; ++k) {
-				int qx = bx + k % side - r, qy = by + k / side - r;
+				const int dx = k % side - r, dy = k / side - r;
+				int qx = bx + dx, qy = by + dy;
 				bool fresh = false;
-				if (p.mode == CUSMIC_WRAP) {
+				const bool reached = wrap || ((dx == 0 || (sides & (dx < 0 ? SIDE_L : SIDE_R))) &&
+							      (dy == 0 || (sides & (dy < 0 ? SIDE_U : SIDE_D))));
+				if (wrap) {
 					qx = (qx % nbx + nbx) % nbx;
 					qy = (qy % nby + nby) % nby;
 				}
 				const int q = block0 + qy * nbx + qx;
-				if (e < n && qx >= 0 && qx < nbx && qy >= 0 && qy < nby)
+				if (e < n && reached && qx >= 0 && qx < nbx && qy >= 0 && qy < nby)
 					fresh = __atomic_load_n(&p.dirty_tag[q], __ATOMIC_RELAXED) != tag &&
 						atomicExch(&p.dirty_tag[q], tag) != tag;
 				lds_append(fresh, list_entry(f, q), out.item, &out.n);

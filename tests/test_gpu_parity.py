@@ -194,6 +194,34 @@ def test_many_passes(cuda):
         assert max(want[3]) > 4, want[3]
 
 
+def test_many_passes_noise_model(cuda):
+    """Under the noise model S' reaches one pixel farther (N is a median of C),
+    so its dirty blocks reach farther than an error-map batch's: frames of both
+    kinds in one batch, many passes, dense tracks."""
+    frames = []
+    for i, shape in enumerate(((120, 140), (70, 190))):
+        d, _, m = scene(shape, 40 + i, nan=1e-3, mask_frac=2e-3, stars=20, tracks=70, max_len=10)
+        frames.append(dict(data=d, mask=m))
+    d, err, m = scene((90, 110), 42, stars=20, tracks=50, max_len=10)
+    frames.append(dict(data=d, error=err, mask=m))
+    for mode in ("reflect", "nearest", "wrap"):
+        say("many passes, noise model", mode)
+        got, want = compare(frames, contrast=1.0, cr_threshold=3.0, neighbor_threshold=1.5,
+                            maxiter=12, border_mode=mode, frame_gain=[2.0, 2.5, 1.0],
+                            frame_readnoise=[5.0, 4.0, 1.0])
+        assert max(want[3]) > 4, want[3]
+
+
+def test_many_passes_interior(cuda):
+    """A frame with interior tiles and blocks on every side of the changes."""
+    d, err, m = scene((256, 300), 50, nan=5e-4, mask_frac=1e-3, stars=40, tracks=300, max_len=10)
+    for mode in ("mirror", "constant"):
+        say("many passes, interior", mode)
+        got, want = compare([dict(data=d, error=err, mask=m)], contrast=1.0, cr_threshold=3.0,
+                            neighbor_threshold=1.5, maxiter=10, border_mode=mode)
+        assert want[3][0] > 4, want[3]
+
+
 def test_tiny_and_odd_shapes(cuda):
     rng = np.random.default_rng(2)
     for shape in ((1, 9), (9, 1), (1, 1), (2, 2), (3, 70), (70, 3), (7, 7), (33, 65), (65, 33)):
