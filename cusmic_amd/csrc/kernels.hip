@@ -1102,8 +1102,12 @@ struct SigShared {
 
 /* This lane's counts of the pixels of a run it owns. */
 struct OwnCounts {
-	int holes, allowed, donors;
-	bool infinite;
+	int holes;            /* holes (skipped tiles) */
+	int nonhole;          /* pixels that are no hole */
+	int masked;           /* masked pixels that are no hole */
+	int nonfinite;        /* pixels that are no hole with a nonfinite C */
+	int masked_nonfinite; /* masked ones of those */
+	bool infinite;        /* an infinite C */
 };
 
 /* Move the buffered seeds of this workgroup to the global list. */
@@ -1128,37 +1132,30 @@ emit_row(SigShared &sh, int &slot, int lane, double t_min, unsigned long long co
 	bool hole)
 {
 	sh.s[slot][lane] = v;
-	unsigned long long m = __ballot(v > t_min) & cols;
-	bool any_bad = __ballot(bad) != 0, any_hole = __ballot(hole) != 0;
-	if (!lane) {
-		sh.screen[slot] = m;
-		sh.bad[slot] = int(any_bad) | int(any_hole) << 1;
-	}
+	const unsigned long long m = __ballot(v > t_min) & cols;
+	const int rowbits = int(__ballot(bad) != 0) | int(__ballot(hole) != 0) << 1;
+	/* every lane stores the same words: without a branch the loads of later
+	 * rows can be issued ahead of this one */
+	sh.screen[slot] = m;
+	sh.bad[slot] = rowbits;
 	slot = slot + 1 == RING ? 0 : slot + 1;
 }
 
-/* The prologue's work at a row of owned pixels (own: this lane's pixel i of
- * frame f belongs to the run): C (unless the caller stores it later), the
- * counts of allowed pixels and finite donors, and EXCL at masked pixels (the
- * flags plane starts zeroed). A hole makes its tile skip, where the holes are
- * flagged, listed and counted. */
+/* The prologue's work at a row of owned pixels (own: this lane's pixel i
+ * belongs to the run): C (unless the caller stores it later) and the counts
+ * behind the per-frame allowed pixels and donors. Branch-free. Masked pixels
+ * are flagged and counted per tile, holes where their tile skips. */
 template <bool BG>
 __device__ __forceinline__ void
 own_row(const Params &p, const Frame &fr, bool own, long long i, double c, bool hole, bool store_c,
 	OwnCounts &acc)
 {
-	if (own) {
-		const uint8_t m = fr.mask ? fr.mask[i] : 0;
-		const bool allow = !hole && !m;
-		if (store_c)
-			p.clean[fr.off + i] = c;
-		if (m)
-			p.flags[fr.off + i] = uint8_t(EXCL);
-		acc.allowed += allow;
-		if (BG) {
-			acc.donors += allow && isfinite(c);
-			acc.infinite |= isinf(c);
-		}
+	if (store_c && own)
+		p.clean[fr.off + i] = c;
+	acc.nonhole += own && !hole;
+	if (BG) {
+		acc.nonfinite += own && !hole && !isfinite(c);
+		acc.infinite |= own && isinf(c);
 	}
 }
 
@@ -1224,45 +1221,62 @@ significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0
 	/* the rows and their one-pixel ring lie inside the frame */
 	const bool inner = X0 - HS - 1 >= 0 && X0 - HS + SW < w && ya - 1 >= 0 && ya + nr < h;
 	if ((!MODEL || fr.error) && inner) {
-		long long o = (long long)ya * w + gx;
-		const double *dp = fr.data + o, *ep = fr.error + o;
-		const double *bp = BG && fr.background ? fr.background + o : nullptr;
+		/* global-space pointers: flat loads would also count against the
+		 * LDS wait counter */
+		using gdouble = const __attribute__((address_space(1))) double;
+		const long long o = (long long)ya * w + gx, W = w;
+		gdouble *dp = (gdouble *)(fr.data + o), *ep = (gdouble *)(fr.error + o);
+		gdouble *bp = BG && fr.background ? (gdouble *)(fr.background + o) : nullptr;
+		/* the samples of row r: the pixel below, left, right, the noise (and
+		 * the background of the first three), fetched PF rows ahead */
+		constexpr int PF = 2;
+		double dn[NR], dl[NR], dr[NR], nz[NR], bn[NR], bl[NR], br[NR];
+#define FETCH_ROW(k)                                                                       \
+	do {                                                                               \
+		const long long ro = (long long)(k) * W;                                   \
+		dn[k] = dp[ro + W];                                                        \
+		dl[k] = dp[ro - 1];                                                        \
+		dr[k] = dp[ro + 1];                                                        \
+		nz[k] = ep[ro];                                                            \
+		bn[k] = bl[k] = br[k] = p.bg;                                              \
+		if (BG && bp) {                                                            \
+			bn[k] = bp[ro + W];                                                \
+			bl[k] = bp[ro - 1];                                                \
+			br[k] = bp[ro + 1];                                                \
+		}                                                                          \
+	} while (0)
 		bool hu = false, hc = false;
-		double qu = input_c<BG>(dp[-w], bp ? bp[-w] : p.bg, has_bg, hu) * 0.25;
+		double qu = input_c<BG>(dp[-W], bp ? bp[-W] : p.bg, has_bg, hu) * 0.25;
 		double cc = input_c<BG>(dp[0], bp ? bp[0] : p.bg, has_bg, hc);
 		double qc = cc * 0.25;
-		/* fully unrolled (constant NR, convergent ballots), so that the rows'
-		 * C stays in registers for the caller to store after the loads */
 #pragma unroll
-		for (int r = 0; r < nr; ++r) {
-			const double dn = dp[w], dl = dp[-1], dr = dp[1], nz = *ep;
-			double bn = p.bg, bl = p.bg, br = p.bg;
-			if (BG && bp) {
-				bn = bp[w];
-				bl = bp[-1];
-				br = bp[1];
-			}
+		for (int r = 0; r < PF && r < NR; ++r)
+			FETCH_ROW(r);
+		/* fully unrolled and branch-free (constant NR, convergent ballots):
+		 * the loads run PF rows ahead, and the rows' C stays in registers for
+		 * the caller to store after the step's loads */
+#pragma unroll
+		for (int r = 0; r < NR; ++r) {
+			if (r + PF < NR)
+				FETCH_ROW(r + PF);
 			bool hd = false, hs = false;
-			const double cd = input_c<BG>(dn, bn, has_bg, hd);
-			const double ql = input_c<BG>(dl, bl, has_bg, hs) * 0.25;
-			const double qr = input_c<BG>(dr, br, has_bg, hs) * 0.25;
+			const double cd = input_c<BG>(dn[r], bn[r], has_bg, hd);
+			const double ql = input_c<BG>(dl[r], bl[r], has_bg, hs) * 0.25;
+			const double qr = input_c<BG>(dr[r], br[r], has_bg, hs) * 0.25;
 			const double qd = cd * 0.25;
 			const int gy = ya + r;
-			own_row<BG>(p, fr, own_lane && gy >= own0 && gy < own1, o, cc, hc, false, acc);
+			own_row<BG>(p, fr, own_lane && gy >= own0 && gy < own1, o + r * W, cc, hc, false, acc);
 			cst[r] = cc;
-			emit_row(sh, slot, lane, t_min, cols, laplacian_q<EXACT, TAME>(qc, qu, qd, ql, qr, false) / (2 * nz),
-				invalid_noise(nz), hu || hc || hd || hs);
+			emit_row(sh, slot, lane, t_min, cols,
+				laplacian_q<EXACT, TAME>(qc, qu, qd, ql, qr, false) / (2 * nz[r]),
+				invalid_noise(nz[r]), hu || hc || hd || hs);
 			qu = qc;
 			qc = qd;
 			cc = cd;
 			hu = hc;
 			hc = hd;
-			dp += w;
-			ep += w;
-			o += w;
-			if (BG && bp)
-				bp += w;
 		}
+#undef FETCH_ROW
 		return true;
 	} else if (!MODEL || fr.error) {
 		bool hu = false, hc = false;
@@ -1362,7 +1376,8 @@ significance_pass(SigShared &sh, const Params &p, int iter)
 		const int w = fr.w, h = fr.h, X0 = tx * TILE_W, Y0 = ty0 * TILE_H, yb = Y0 - HS;
 		const int ncols = min(TILE_W, w - X0), own1 = min(Y0 + TILE_H * K, h);
 		const unsigned long long cols = ((1ull << ncols) - 1) << HS;
-		OwnCounts acc{0, 0, 0, false};
+		const bool has_bg = BG && (fr.background || !isnan(p.bg));
+		OwnCounts acc{0, 0, 0, 0, 0, false};
 
 		for (int s = 0; s < K; ++s) {
 			/* per-step opaque thread index, as in significance_rows */
@@ -1393,6 +1408,28 @@ significance_pass(SigShared &sh, const Params &p, int iter)
 				sh.na = 0;
 			}
 			__syncthreads();
+
+			/* EXCL at the tile's masked pixels (the flags plane starts zeroed;
+			 * holes are flagged where their tile skips) */
+			if (fr.mask) {
+				for (int k = tid; k < TILE_W * TILE_H; k += BLOCK) {
+					const int row = k / TILE_W, col = k % TILE_W;
+					if (row >= nrows || col >= ncols)
+						continue;
+					const int i = (ty + row) * w + X0 + col;
+					const double d = fr.data[i];
+					if (!fr.mask[i] || !isfinite(d))
+						continue;
+					p.flags[fr.off + i] = uint8_t(EXCL);
+					++acc.masked;
+					if (BG) {
+						bool hh = false;
+						const double c = input_c<BG>(d, fr.background ? fr.background[i] : p.bg,
+							has_bg, hh);
+						acc.masked_nonfinite += !isfinite(c);
+					}
+				}
+			}
 
 			/* 2. out-of-frame slots the medians read: the mapped in-frame slot
 			 * (median border mode), or 0 for CONSTANT */
@@ -1501,7 +1538,8 @@ significance_pass(SigShared &sh, const Params &p, int iter)
 		/* the run's counts: one LDS atomic per wave, one global atomic per
 		 * count and run */
 		{
-			int c3[3] = {acc.holes, acc.allowed, BG ? acc.donors : acc.allowed};
+			const int allowed = acc.nonhole - acc.masked;
+			int c3[3] = {acc.holes, allowed, allowed - (BG ? acc.nonfinite - acc.masked_nonfinite : 0)};
 #pragma unroll
 			for (int k = 0; k < 3; ++k) {
 				for (int o = 32; o; o >>= 1)
