@@ -1110,18 +1110,30 @@ struct OwnCounts {
 	bool infinite;        /* an infinite C */
 };
 
+/* A workgroup barrier that orders LDS only. The first pass's global stores
+ * are read by later kernels alone, and no two of its waves store to the same
+ * address, so its barriers need not wait for them (a full barrier would wait
+ * for every store in flight). */
+__device__ __forceinline__ void
+lds_barrier()
+{
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+	__builtin_amdgcn_s_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 /* Move the buffered seeds of this workgroup to the global list. */
 __device__ void
 flush_seeds(SigShared &sh, unsigned long long *list, unsigned int *count)
 {
-	__syncthreads();
+	lds_barrier();
 	const int n = sh.nseed;
 	if (!threadIdx.x && n)
 		sh.flush_base = atomicAdd(count, (unsigned)n);
-	__syncthreads();
+	lds_barrier();
 	for (int k = threadIdx.x; k < n; k += BLOCK)
 		list[sh.flush_base + k] = sh.seeds[k];
-	__syncthreads();
+	lds_barrier();
 	if (!threadIdx.x)
 		sh.nseed = 0;
 }
@@ -1357,14 +1369,14 @@ significance_pass(SigShared &sh, const Params &p, int iter)
 	}
 
 	for (;;) {
-		__syncthreads();
+		lds_barrier();
 		if (!threadIdx.x) {
 			unsigned q = next;
 			sh.item = q < limit ? (int)q : -1;
 			if (q < limit)
 				next = atomicAdd(&p.queue[iter], 1u);
 		}
-		__syncthreads();
+		lds_barrier();
 		const int item = __builtin_amdgcn_readfirstlane(sh.item);
 		if (item < 0)
 			break;
@@ -1395,7 +1407,7 @@ significance_pass(SigShared &sh, const Params &p, int iter)
 				deferred = significance_rows<MODEL, EXACT, BG, STEP_ROWS / 4>(sh, p, fr, f, X0, yb,
 					r0, t_min, cols, Y0, own1, acc, cst);
 			} else {
-				__syncthreads(); /* the last tile's medians are done with the ring */
+				lds_barrier(); /* the last tile's medians are done with the ring */
 				r0 = TILE_H * s + 2 * HS + wave * (TILE_H / 4);
 				nr = TILE_H / 4;
 				deferred = significance_rows<MODEL, EXACT, BG, TILE_H / 4>(sh, p, fr, f, X0, yb,
@@ -1407,29 +1419,7 @@ significance_pass(SigShared &sh, const Params &p, int iter)
 				work += (unsigned long long)(ncols * nrows);
 				sh.na = 0;
 			}
-			__syncthreads();
-
-			/* EXCL at the tile's masked pixels (the flags plane starts zeroed;
-			 * holes are flagged where their tile skips) */
-			if (fr.mask) {
-				for (int k = tid; k < TILE_W * TILE_H; k += BLOCK) {
-					const int row = k / TILE_W, col = k % TILE_W;
-					if (row >= nrows || col >= ncols)
-						continue;
-					const int i = (ty + row) * w + X0 + col;
-					const double d = fr.data[i];
-					if (!fr.mask[i] || !isfinite(d))
-						continue;
-					p.flags[fr.off + i] = uint8_t(EXCL);
-					++acc.masked;
-					if (BG) {
-						bool hh = false;
-						const double c = input_c<BG>(d, fr.background ? fr.background[i] : p.bg,
-							has_bg, hh);
-						acc.masked_nonfinite += !isfinite(c);
-					}
-				}
-			}
+			lds_barrier();
 
 			/* 2. out-of-frame slots the medians read: the mapped in-frame slot
 			 * (median border mode), or 0 for CONSTANT */
@@ -1445,7 +1435,7 @@ significance_pass(SigShared &sh, const Params &p, int iter)
 						v = sh.s[(border_index(y, h, mode) - yb) % RING][border_index(x, w, mode) - (X0 - HS)];
 					sh.s[(r_lo + sr) % RING][sx] = v;
 				}
-				__syncthreads();
+				lds_barrier();
 			}
 
 			/* 3. screened pixels of the tile: S' <= S, so only S > min(t, t_n)
@@ -1488,7 +1478,32 @@ significance_pass(SigShared &sh, const Params &p, int iter)
 					lds_append(((m >> lane) & 1) != 0, (uint16_t)((row << 6) | lane), sh.alist, &sh.na);
 				}
 			}
-			__syncthreads();
+			/* EXCL at the tile's masked pixels (the flags plane starts zeroed;
+			 * holes are flagged above). A screened pixel's flags are written
+			 * once, with SCR, by the medians below. */
+			if (fr.mask) {
+				for (int k = tid; k < TILE_W * TILE_H; k += BLOCK) {
+					const int row = k / TILE_W, col = k % TILE_W;
+					if (row >= nrows || col >= ncols)
+						continue;
+					const int i = (ty + row) * w + X0 + col;
+					const double d = fr.data[i];
+					if (!fr.mask[i] || !isfinite(d))
+						continue;
+					const bool screened = !skip && (unsafe ||
+						((sh.screen[(r_lo + HS + row) % RING] >> (col + HS)) & 1));
+					if (!screened)
+						p.flags[fr.off + i] = uint8_t(EXCL);
+					++acc.masked;
+					if (BG) {
+						bool hh = false;
+						const double c = input_c<BG>(d, fr.background ? fr.background[i] : p.bg,
+							has_bg, hh);
+						acc.masked_nonfinite += !isfinite(c);
+					}
+				}
+			}
+			lds_barrier();
 
 			/* the deferred C of this wave's rows: its latency overlaps the
 			 * medians below */
@@ -1530,7 +1545,7 @@ significance_pass(SigShared &sh, const Params &p, int iter)
 					seed = sp > t_cr;
 				}
 				lds_append(seed, list_entry(f, i), sh.seeds, &sh.nseed);
-				__syncthreads();
+				lds_barrier();
 				if (sh.nseed + BLOCK > SEED_BUF)
 					flush_seeds(sh, p.list_b, seeds);
 			}
@@ -1555,7 +1570,7 @@ significance_pass(SigShared &sh, const Params &p, int iter)
 					atomicOr(&p.status[1], 1);
 			}
 		}
-		__syncthreads();
+		lds_barrier();
 		if (!threadIdx.x) {
 			if (sh.own[0])
 				atomicAdd(&p.counts[f].holes, sh.own[0]);
