@@ -248,10 +248,11 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 
 	/* one scratch block: frame table | counters (zeroed) | flags | list */
 	size_t n_counts = nf * (sizeof(Counts) / 4);
+	size_t n_shard = nf * size_t(SHARDS * SHARD_FIELDS);
 	size_t n_todo = size_t(std::max(maxiter, 1)) * nf;
 	size_t n_queue = size_t(std::max(maxiter, 1));
 	size_t n_nlist = 4 + 9 * size_t(std::max(maxiter, 1));
-	size_t words = n_counts + n_todo + n_queue + n_nlist + 2 + 4 * size_t(nblocks) + size_t(ntiles);
+	size_t words = n_counts + n_shard + n_todo + n_queue + n_nlist + 2 + 4 * size_t(nblocks) + size_t(ntiles);
 	size_t off_frames = 0;
 	size_t off_words = align_up(nf * sizeof(Frame), 256);
 	size_t off_dlist = align_up(off_words + words * 4, 256);
@@ -289,8 +290,9 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	p.donors = donors;
 	unsigned int *w32 = reinterpret_cast<unsigned int *>(base + off_words);
 	p.counts = reinterpret_cast<Counts *>(w32);
-	p.todo = reinterpret_cast<int *>(w32 + n_counts);
-	p.queue = w32 + n_counts + n_todo;
+	p.shard = reinterpret_cast<int *>(w32 + n_counts);
+	p.todo = reinterpret_cast<int *>(w32 + n_counts + n_shard);
+	p.queue = w32 + n_counts + n_shard + n_todo;
 	p.nlist = p.queue + n_queue;
 	p.status = reinterpret_cast<int *>(p.nlist + n_nlist);
 	p.block_pass = reinterpret_cast<unsigned int *>(p.status + 2);
@@ -358,6 +360,8 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	const int sig_resident = cus * std::max(sig_per_cu, 1);
 	hipLaunchKernelGGL(sig, dim3(unsigned(std::min(nruns, sig_resident))), dim3(BLOCK), 0, stream, p, 0);
 	tm.mark(CUSMIC_PROFILE_FIRST_PASS_MS);
+	const unsigned fold_grid = unsigned((nf + 255) / 256);
+	hipLaunchKernelGGL(fold_counts, dim3(fold_grid), dim3(256), 0, stream, p, 0, 1);
 	unsigned replace_grid = unsigned(cus) * 8;
 	hipLaunchKernelGGL(replace_pass, dim3(replace_grid), dim3(BLOCK), 0, stream, p, 0, 1);
 	hipLaunchKernelGGL(replace_widened, dim3(cus), dim3(BLOCK), 0, stream, p, 0, 1);
@@ -378,6 +382,7 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 		hipLaunchKernelGGL(fine, dim3(list_grid), dim3(BLOCK), 0, stream, p, it);
 		hipLaunchKernelGGL(grow_first, dim3(list_grid), dim3(BLOCK), 0, stream, p, it);
 		hipLaunchKernelGGL(grow_second, dim3(list_grid), dim3(BLOCK), 0, stream, p, it);
+		hipLaunchKernelGGL(fold_counts, dim3(fold_grid), dim3(256), 0, stream, p, it, 0);
 		if (it > 0)
 			hipLaunchKernelGGL(redo_scan, dim3(list_grid), dim3(BLOCK), 0, stream, p, it);
 		tm.mark(CUSMIC_PROFILE_DETECT_MS);

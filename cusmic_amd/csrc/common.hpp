@@ -74,6 +74,13 @@ struct Frame {
 /* Pointwise passes stream frames in chunks of CHUNK consecutive pixels. */
 constexpr int CHUNK = 8192;
 
+/* Per-frame counters are updated in SHARDS copies, one per group of
+ * workgroups (blockIdx.x % SHARDS), and folded into the totals by a small
+ * kernel: many workgroups adding to one address serialise at the memory side
+ * (hundreds of nanoseconds each on this device). */
+constexpr int SHARDS = 32;
+enum { SHARD_HOLES, SHARD_ALLOWED, SHARD_DONORS, SHARD_FRESH, SHARD_FIELDS };
+
 /* Per-frame counters of one call, zeroed at its start. */
 struct Counts {
 	int holes, allowed, donors, error;
@@ -97,6 +104,7 @@ struct Params {
 	int32_t *donors;                 /* 2 per pixel or null */
 	unsigned long long *list;        /* hole list, then the flagged-pixel list */
 	Counts *counts;                  /* per frame */
+	int *shard;                      /* [nf][SHARDS][SHARD_FIELDS] pending counts */
 	int *todo;                       /* [maxiter][nf] new pixels per pass */
 	unsigned int *queue;             /* [maxiter] tile queue heads */
 	unsigned int *nlist;             /* [0] list length, [1] hole count,

@@ -145,6 +145,27 @@ zero_groups(uint4 *groups, size_t n)
 		groups[i] = make_uint4(0, 0, 0, 0);
 }
 
+/* Fold the pending per-frame count shards into the totals (and, after a
+ * detection pass, the pass's new-pixel count) and clear them. */
+__global__ void
+fold_counts(Params p, int iter, int first)
+{
+	const int f = blockIdx.x * blockDim.x + threadIdx.x;
+	if (f >= p.nf)
+		return;
+	int d[SHARD_FIELDS] = {};
+	int *sh = p.shard + (size_t)f * SHARDS * SHARD_FIELDS;
+	for (int s = 0; s < SHARDS * SHARD_FIELDS; ++s) {
+		d[s % SHARD_FIELDS] += sh[s];
+		sh[s] = 0;
+	}
+	p.counts[f].holes += d[SHARD_HOLES];
+	p.counts[f].allowed += d[SHARD_ALLOWED];
+	p.counts[f].donors += d[SHARD_DONORS];
+	if (!first)
+		p.todo[iter * p.nf + f] = d[SHARD_FRESH];
+}
+
 /* Donor records start as (-1, -1): no donor. */
 __global__ void
 reset_donors(int32_t *donors, size_t n)
@@ -362,8 +383,8 @@ rank_key(double v, int r, int cnt_neg0)
 /* Record that pixel i of frame f changed (changed = this lane's pixel did)
  * in the replacement before pass next: the first such pixel of a block puts
  * the block on that pass's changed list (one atomic per wave). */
-__device__ __forceinline__ void
-mark_dirty(const Params &p, bool changed, int f, int i, int next)
+__device__ __forceinline__ bool
+changed_block(const Params &p, bool changed, int f, int i, int next, unsigned long long &entry)
 {
 	const unsigned tag = pass_stamp(next);
 	unsigned b = 0;
@@ -379,7 +400,18 @@ mark_dirty(const Params &p, bool changed, int f, int i, int next)
 		fresh = __atomic_load_n(&p.changed_tag[b], __ATOMIC_RELAXED) != tag &&
 			atomicExch(&p.changed_tag[b], tag) != tag;
 	}
-	list_push(fresh, list_entry(f, (int)b), p.changed_list, &p.nlist[nl_changed(p, next)]);
+	entry = list_entry(f, (int)b);
+	return fresh;
+}
+
+/* Record that pixel i of frame f changed (changed: this lane's pixel did):
+ * its block joins the changed list of pass next once (whole wave). */
+__device__ __forceinline__ void
+mark_dirty(const Params &p, bool changed, int f, int i, int next)
+{
+	unsigned long long entry;
+	const bool fresh = changed_block(p, changed, f, i, next, entry);
+	list_push(fresh, entry, p.changed_list, &p.nlist[nl_changed(p, next)]);
 }
 
 /* Blocks of pass iter: the neighbours of a changed block on the sides its
@@ -812,6 +844,12 @@ replace_allowed(const Params &p, int f, int iter, int fill)
 __global__ __launch_bounds__(BLOCK) void
 replace_pass(Params p, int iter, int fill)
 {
+	/* changed blocks, flushed when the next round might not fit */
+	__shared__ BlockList<2 * BLOCK> changed_out;
+	if (!threadIdx.x)
+		changed_out.n = 0;
+	__syncthreads();
+	const bool marking = !fill && iter + 1 < p.maxiter;
 	const unsigned start = fill ? 0 : p.nlist[nl_start(p, iter)];
 	const unsigned n_new = fill ? p.nlist[1] : p.nlist[0] - start;
 	const unsigned n = n_new + (fill ? 0 : p.nlist[nl_redo(p, iter)]);
@@ -837,9 +875,17 @@ replace_pass(Params p, int iter, int fill)
 			}
 		}
 		list_push(widen, entry, p.list_c, wide);
-		if (!fill && iter + 1 < p.maxiter)
-			mark_dirty(p, changed, f, i, iter + 1);
+		if (marking) {
+			unsigned long long block_entry;
+			const bool fresh = changed_block(p, changed, f, i, iter + 1, block_entry);
+			lds_append(fresh, block_entry, changed_out.item, &changed_out.n);
+			__syncthreads();
+			if (changed_out.n > BLOCK)
+				block_list_flush(changed_out, p.changed_list, &p.nlist[nl_changed(p, iter + 1)]);
+		}
 	}
+	if (marking)
+		block_list_flush(changed_out, p.changed_list, &p.nlist[nl_changed(p, iter + 1)]);
 }
 
 /* Windows wider than 5 x 5, one wave per pixel: this pass's widened pixels,
@@ -1572,12 +1618,13 @@ significance_pass(SigShared &sh, const Params &p, int iter)
 		}
 		lds_barrier();
 		if (!threadIdx.x) {
+			int *shard = p.shard + ((size_t)f * SHARDS + blockIdx.x % SHARDS) * SHARD_FIELDS;
 			if (sh.own[0])
-				atomicAdd(&p.counts[f].holes, sh.own[0]);
+				atomicAdd(shard + SHARD_HOLES, sh.own[0]);
 			if (sh.own[1])
-				atomicAdd(&p.counts[f].allowed, sh.own[1]);
+				atomicAdd(shard + SHARD_ALLOWED, sh.own[1]);
 			if (sh.own[2])
-				atomicAdd(&p.counts[f].donors, sh.own[2]);
+				atomicAdd(shard + SHARD_DONORS, sh.own[2]);
 			sh.own[0] = sh.own[1] = sh.own[2] = 0;
 		}
 	}
@@ -1908,7 +1955,7 @@ fine_structure(const Params &p, const Frame &fr, int x, int y)
 __global__ __launch_bounds__(BLOCK) void
 fine(Params p, int iter)
 {
-	__shared__ BlockList<BLOCK> out;
+	__shared__ BlockList<2 * BLOCK> out; /* flushed when the next round might not fit */
 	const unsigned n = p.nlist[nl_seed(p, iter)];
 	unsigned int *count = &p.nlist[nl_cand(p, iter)];
 	if (!threadIdx.x)
@@ -1928,8 +1975,11 @@ fine(Params p, int iter)
 		}
 		__syncthreads();
 		lds_append(cand, entry, out.item, &out.n);
-		block_list_flush(out, p.list_c, count);
+		__syncthreads();
+		if (out.n > BLOCK)
+			block_list_flush(out, p.list_c, count);
 	}
+	block_list_flush(out, p.list_c, count);
 }
 
 /* S' at pixel (x, y) of a frame if the last pass that processed the pixel
@@ -2035,8 +2085,9 @@ grow_second(Params p, int iter)
 			}
 			lds_append(fresh, list_entry(f, q), out.item, &out.n);
 		}
-		wave_frame_sum(fresh_n, f, p.todo + iter * p.nf, 1);
-		wave_frame_sum(-lost_n, f, &p.counts[0].donors, int(sizeof(Counts) / sizeof(int)));
+		int *shard = p.shard + (blockIdx.x % SHARDS) * SHARD_FIELDS;
+		wave_frame_sum(fresh_n, f, shard + SHARD_FRESH, SHARDS * SHARD_FIELDS);
+		wave_frame_sum(-lost_n, f, shard + SHARD_DONORS, SHARDS * SHARD_FIELDS);
 		block_list_flush(out, p.list, &p.nlist[0]);
 	}
 }
