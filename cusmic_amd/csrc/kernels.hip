@@ -1671,6 +1671,8 @@ struct BlockShared {
 	double s[BLOCK / 64][BR * BR];
 	unsigned long long seeds[BLOCK / 64][WAVE_SEEDS];
 	int nseed[BLOCK / 64];
+	unsigned base;
+	unsigned long long work;
 };
 
 
@@ -1715,7 +1717,9 @@ significance_blocks_pass(BlockShared &sh, const Params &p, int iter)
 	unsigned long long work = 0;
 	if (!lane)
 		*nseed = 0;
-	__builtin_amdgcn_wave_barrier();
+	if (!threadIdx.x)
+		sh.work = 0;
+	__syncthreads();
 	const unsigned waves = gridDim.x * (BLOCK / 64);
 	unsigned e = blockIdx.x * (BLOCK / 64) + wave;
 	/* the next block's list entry is loaded while this one is processed */
@@ -1796,9 +1800,25 @@ significance_blocks_pass(BlockShared &sh, const Params &p, int iter)
 		if (*nseed > WAVE_SEEDS - 64)
 			flush_wave_seeds(buf, nseed, p.list_b, seeds);
 	}
-	flush_wave_seeds(buf, nseed, p.list_b, seeds);
+	/* the workgroup's remaining seeds and its pixel count: one global atomic
+	 * each for all four waves */
 	if (!lane && work)
-		atomicAdd(&p.work[iter], work);
+		atomicAdd(&sh.work, work);
+	__syncthreads();
+	if (!threadIdx.x) {
+		unsigned total = 0;
+		for (int k = 0; k < BLOCK / 64; ++k)
+			total += sh.nseed[k];
+		sh.base = total ? atomicAdd(seeds, total) : 0;
+		if (sh.work)
+			atomicAdd(&p.work[iter], sh.work);
+	}
+	__syncthreads();
+	unsigned at = sh.base;
+	for (int k = 0; k < wave; ++k)
+		at += sh.nseed[k];
+	for (int j = lane; j < *nseed; j += 64)
+		p.list_b[at + j] = buf[j];
 }
 
 template <bool MODEL, bool EXACT>
