@@ -1132,6 +1132,24 @@ med_input(const Frame &fr, double bg, bool has_bg, int x, int y, int mode, bool 
 	return bx < 0 || by < 0 ? 0.0 : input_at<BG>(fr, bg, has_bg, (long long)by * fr.w + bx, hole);
 }
 
+/* Lane i takes lane i - 1's value (wave_shr:1); lane 0 keeps fill. */
+__device__ __forceinline__ double
+wave_shift_right(double v, double fill)
+{
+	const int lo = __builtin_amdgcn_update_dpp(__double2loint(fill), __double2loint(v), 0x138, 0xf, 0xf, false);
+	const int hi = __builtin_amdgcn_update_dpp(__double2hiint(fill), __double2hiint(v), 0x138, 0xf, 0xf, false);
+	return __hiloint2double(hi, lo);
+}
+
+/* Lane i takes lane i + 1's value (wave_shl:1); lane 63 keeps fill. */
+__device__ __forceinline__ double
+wave_shift_left(double v, double fill)
+{
+	const int lo = __builtin_amdgcn_update_dpp(__double2loint(fill), __double2loint(v), 0x130, 0xf, 0xf, false);
+	const int hi = __builtin_amdgcn_update_dpp(__double2hiint(fill), __double2hiint(v), 0x130, 0xf, 0xf, false);
+	return __hiloint2double(hi, lo);
+}
+
 constexpr int SEED_BUF = 512; /* seeds buffered per workgroup between flushes */
 
 struct SigShared {
@@ -1285,22 +1303,24 @@ significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0
 		const long long o = (long long)ya * w + gx, W = w;
 		gdouble *dp = (gdouble *)(fr.data + o), *ep = (gdouble *)(fr.error + o);
 		gdouble *bp = BG && fr.background ? (gdouble *)(fr.background + o) : nullptr;
-		/* the samples of row r: the pixel below, left, right, the noise (and
-		 * the background of the first three), fetched PF rows ahead */
+		/* the left and right samples of a row are the neighbouring lanes'
+		 * centres (a wave shift); lanes 0 and 63 load theirs, the others
+		 * reload their centre in the same instruction */
+		const int eo = lane == 0 ? -1 : lane == 63 ? 1 : 0;
+		/* the samples of row r: the pixel below, the edge sample and the
+		 * noise (and their background), fetched PF rows ahead */
 		constexpr int PF = 2;
-		double dn[NR], dl[NR], dr[NR], nz[NR], bn[NR], bl[NR], br[NR];
+		double dn[NR], de[NR], nz[NR], bn[NR], be[NR];
 #define FETCH_ROW(k)                                                                       \
 	do {                                                                               \
 		const long long ro = (long long)(k) * W;                                   \
 		dn[k] = dp[ro + W];                                                        \
-		dl[k] = dp[ro - 1];                                                        \
-		dr[k] = dp[ro + 1];                                                        \
+		de[k] = dp[ro + eo];                                                       \
 		nz[k] = ep[ro];                                                            \
-		bn[k] = bl[k] = br[k] = p.bg;                                              \
+		bn[k] = be[k] = p.bg;                                                      \
 		if (BG && bp) {                                                            \
 			bn[k] = bp[ro + W];                                                \
-			bl[k] = bp[ro - 1];                                                \
-			br[k] = bp[ro + 1];                                                \
+			be[k] = bp[ro + eo];                                               \
 		}                                                                          \
 	} while (0)
 		bool hu = false, hc = false;
@@ -1319,8 +1339,8 @@ significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0
 				FETCH_ROW(r + PF);
 			bool hd = false, hs = false;
 			const double cd = input_c<BG>(dn[r], bn[r], has_bg, hd);
-			const double ql = input_c<BG>(dl[r], bl[r], has_bg, hs) * 0.25;
-			const double qr = input_c<BG>(dr[r], br[r], has_bg, hs) * 0.25;
+			const double qe = input_c<BG>(de[r], be[r], has_bg, hs) * 0.25;
+			const double ql = wave_shift_right(qc, qe), qr = wave_shift_left(qc, qe);
 			const double qd = cd * 0.25;
 			const int gy = ya + r;
 			own_row<BG>(p, fr, own_lane && gy >= own0 && gy < own1, o + r * W, cc, hc, false, acc);
