@@ -232,6 +232,34 @@ def main():
         log(f"host call: {host_ms:.3f} ms")
         del d, e, out_h, mask_h
 
+        # complete calls on one 512x512 frame with an error map of 5 and
+        # settings (1, 5, 5), maxiter 4, mirror: the shape and settings of the
+        # reference's published complete-call runs (README.md:123-140), on a
+        # synthetic scene; 4 warmups, median of 16 (bench/bench.cu:204-257)
+        d, _ = scenes.frame((512, 512), 11, "cpu")
+        d = np.ascontiguousarray(d.numpy())
+        e = np.full_like(d, 5.0)
+        out_h = np.empty_like(d)
+        mask_h = np.empty(d.shape, np.uint8)
+        img = _lib.CusmicImage(d.ctypes.data, e.ctypes.data, None, None, None, None, 512, 512)
+        opts = _lib.default_options()
+        opts.contrast, opts.cr_threshold, opts.neighbor_threshold = 1.0, 5.0, 5.0
+        for _ in range(4):
+            host_call()
+        samples = []
+        for _ in range(16):
+            t0 = time.perf_counter()
+            host_call()
+            samples.append(1e3 * (time.perf_counter() - t0))
+        ms512 = float(np.median(samples))
+        result["host_call_512"] = {"value": round(0.262144 / (ms512 / 1e3), 2), "unit": "Mpix/s",
+                                   "ms_per_frame": round(ms512, 3),
+                                   "frame": "512x512 synthetic, error map 5, settings (1, 5, 5), maxiter 4",
+                                   "note": "complete host calls (validation, allocation, copies); "
+                                           "median of 16 after 4 warmups"}
+        log(f"host call 512x512: {ms512:.3f} ms")
+        del d, e, out_h, mask_h
+
     if not args.no_backward:
         log("forward+backward, 8 x 1024^2 noise-model frames")
         from cusmic_amd import autograd
