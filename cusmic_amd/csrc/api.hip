@@ -360,7 +360,7 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	const int sig_resident = cus * std::max(sig_per_cu, 1);
 	hipLaunchKernelGGL(sig, dim3(unsigned(std::min(nruns, sig_resident))), dim3(BLOCK), 0, stream, p, 0);
 	tm.mark(CUSMIC_PROFILE_FIRST_PASS_MS);
-	const unsigned fold_grid = unsigned((nf + 255) / 256);
+	const unsigned fold_grid = unsigned((nf * SHARDS + 255) / 256);
 	hipLaunchKernelGGL(fold_counts, dim3(fold_grid), dim3(256), 0, stream, p, 0, 1);
 	unsigned replace_grid = unsigned(cus) * 8;
 	hipLaunchKernelGGL(replace_pass, dim3(replace_grid), dim3(BLOCK), 0, stream, p, 0, 1);

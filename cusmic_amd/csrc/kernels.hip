@@ -146,24 +146,33 @@ zero_groups(uint4 *groups, size_t n)
 }
 
 /* Fold the pending per-frame count shards into the totals (and, after a
- * detection pass, the pass's new-pixel count) and clear them. */
+ * detection pass, the pass's new-pixel count) and clear them: SHARDS lanes
+ * per frame, one shard each, summed across the lanes. */
 __global__ void
 fold_counts(Params p, int iter, int first)
 {
-	const int f = blockIdx.x * blockDim.x + threadIdx.x;
-	if (f >= p.nf)
-		return;
+	static_assert(SHARDS == 32, "a frame's shards fill half a wave");
+	const int t = blockIdx.x * blockDim.x + threadIdx.x, f = t / SHARDS;
 	int d[SHARD_FIELDS] = {};
-	int *sh = p.shard + (size_t)f * SHARDS * SHARD_FIELDS;
-	for (int s = 0; s < SHARDS * SHARD_FIELDS; ++s) {
-		d[s % SHARD_FIELDS] += sh[s];
-		sh[s] = 0;
+	if (f < p.nf) {
+		int *sh = p.shard + (size_t)t * SHARD_FIELDS;
+#pragma unroll
+		for (int k = 0; k < SHARD_FIELDS; ++k) {
+			d[k] = sh[k];
+			sh[k] = 0;
+		}
 	}
-	p.counts[f].holes += d[SHARD_HOLES];
-	p.counts[f].allowed += d[SHARD_ALLOWED];
-	p.counts[f].donors += d[SHARD_DONORS];
-	if (!first)
-		p.todo[iter * p.nf + f] = d[SHARD_FRESH];
+#pragma unroll
+	for (int k = 0; k < SHARD_FIELDS; ++k)
+		for (int o = SHARDS / 2; o; o >>= 1)
+			d[k] += __shfl_xor(d[k], o, 64);
+	if (f < p.nf && !(t % SHARDS)) {
+		p.counts[f].holes += d[SHARD_HOLES];
+		p.counts[f].allowed += d[SHARD_ALLOWED];
+		p.counts[f].donors += d[SHARD_DONORS];
+		if (!first)
+			p.todo[iter * p.nf + f] = d[SHARD_FRESH];
+	}
 }
 
 /* Donor records start as (-1, -1): no donor. */
