@@ -174,6 +174,7 @@ def main():
                       ("prologue_ms", "first_pass_ms", "significance_ms", "detect_ms", "replace_ms",
                        "epilogue_ms")},
         "lists_per_call": {k: int(profiles[0][k]) for k in ("seeds", "candidates", "grown", "widened")},
+        "block_pass_pixels": int(mean("detect_pixels") - total),
     }
 
     if not args.no_per_frame:

@@ -252,11 +252,11 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	size_t n_todo = size_t(std::max(maxiter, 1)) * nf;
 	size_t n_queue = size_t(std::max(maxiter, 1));
 	size_t n_nlist = 4 + 9 * size_t(std::max(maxiter, 1));
-	size_t words = n_counts + n_shard + n_todo + n_queue + n_nlist + 2 + 4 * size_t(nblocks) + size_t(ntiles);
+	size_t words = n_counts + n_shard + n_todo + n_queue + n_nlist + 2 + 5 * size_t(nblocks) + size_t(ntiles);
 	size_t off_frames = 0;
 	size_t off_words = align_up(nf * sizeof(Frame), 256);
 	size_t off_dlist = align_up(off_words + words * 4, 256);
-	size_t off_clist = align_up(off_dlist + size_t(nblocks) * 8, 256);
+	size_t off_clist = align_up(off_dlist + size_t(nblocks) * 16, 256); /* recompute + reseed */
 	size_t off_work = align_up(off_clist + size_t(nblocks) * 8, 256);
 	size_t off_cframe = align_up(off_work + n_queue * 8, 256);
 	size_t off_rframe = align_up(off_cframe + size_t(nchunks) * 4, 256);
@@ -299,7 +299,8 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	p.changed_tag = p.block_pass + nblocks;
 	p.dirty_tag = p.changed_tag + nblocks;
 	p.changed_sides = p.dirty_tag + nblocks;
-	p.tile_skip = p.changed_sides + nblocks;
+	p.reseed_tag = p.changed_sides + nblocks;
+	p.tile_skip = p.reseed_tag + nblocks;
 	p.changed_list = reinterpret_cast<unsigned long long *>(base + off_clist);
 	p.nblocks = nblocks;
 	p.dirty_list = reinterpret_cast<unsigned long long *>(base + off_dlist);
@@ -333,6 +334,7 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 		bg_any |= ims[f].background != nullptr;
 	}
 	p.reach = model ? REACH : REACH_MAPS;
+	p.reach_s = model ? REACH_S : REACH_S_MAPS;
 	timer tm(stream, profile != nullptr);
 	tm.mark(-1);
 	hipLaunchKernelGGL(zero_words, dim3(unsigned(std::min<size_t>((words + 255) / 256, 4096))),

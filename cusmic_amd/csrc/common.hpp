@@ -55,9 +55,19 @@ constexpr double SIGNAL_FLOOR = 1e-5; /* src/filters.cuh:22 */
  * either. */
 constexpr int REACH_MAPS = 5;
 constexpr int REACH = 6;
+/* Reach of a change on S' itself: blocks within it recompute S'; blocks only
+ * within the detection reach keep their S' and re-emit their seeds. */
+constexpr int REACH_S_MAPS = 3;
+constexpr int REACH_S = 4;
 
-/* Sides of a block that a change at in-block offset (ox, oy) reaches. */
+/* Sides of a block that a change at in-block offset (ox, oy) reaches: bits
+ * 0-3 for the detection reach, bits 4-7 for the S' reach. */
 constexpr unsigned SIDE_L = 1, SIDE_R = 2, SIDE_U = 4, SIDE_D = 8;
+constexpr int SIDES_S = 4; /* shift of the S' reach's sides */
+
+/* Dirty-list entries of blocks that only re-emit seeds carry this bit in their
+ * block index. */
+constexpr unsigned RESEED = 1u << 31;
 
 struct Frame {
 	const double *data, *error, *gain, *readnoise, *background;
@@ -89,7 +99,7 @@ struct Counts {
 struct Params {
 	const Frame *frames;
 	int nf, ntiles, maxiter, mode, laplacian;
-	int reach;                       /* REACH_MAPS or REACH */
+	int reach, reach_s;              /* REACH_MAPS or REACH, REACH_S_MAPS or REACH_S */
 	double contrast, threshold, neighbor, bg;  /* bg: scalar background or NaN */
 	double *clean;
 	uint8_t *flags;
@@ -120,6 +130,8 @@ struct Params {
 					  * the block */
 	unsigned int *changed_sides;     /* [nblocks] SIDE_* bits the block's
 					  * changes reach; reset when expanded */
+	unsigned int *reseed_tag;        /* [nblocks] stamp of the last pass that
+					  * listed the block to re-emit its seeds */
 	unsigned int *dirty_tag;         /* [nblocks] stamp of the last pass that
 					  * listed the block as dirty */
 	unsigned long long *changed_list; /* frame << 32 | block, per pass */

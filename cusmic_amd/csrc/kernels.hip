@@ -402,8 +402,11 @@ changed_block(const Params &p, bool changed, int f, int i, int next, unsigned lo
 		const Frame fr = p.frames[f];
 		const int x = i % fr.w, y = i / fr.w, ox = x % BLK, oy = y % BLK;
 		b = unsigned(fr.block0 + (y / BLK) * fr.blocks_x + x / BLK);
-		const unsigned sides = (ox < p.reach ? SIDE_L : 0u) | (ox >= BLK - p.reach ? SIDE_R : 0u) |
-				       (oy < p.reach ? SIDE_U : 0u) | (oy >= BLK - p.reach ? SIDE_D : 0u);
+		auto sides_of = [&](int r) {
+			return (ox < r ? SIDE_L : 0u) | (ox >= BLK - r ? SIDE_R : 0u) | (oy < r ? SIDE_U : 0u) |
+			       (oy >= BLK - r ? SIDE_D : 0u);
+		};
+		const unsigned sides = sides_of(p.reach) | sides_of(p.reach_s) << SIDES_S;
 		if ((__atomic_load_n(&p.changed_sides[b], __ATOMIC_RELAXED) & sides) != sides)
 			atomicOr(&p.changed_sides[b], sides);
 		fresh = __atomic_load_n(&p.changed_tag[b], __ATOMIC_RELAXED) != tag &&
@@ -425,10 +428,12 @@ mark_dirty(const Params &p, bool changed, int f, int i, int next)
 
 /* Blocks of pass iter: the neighbours of a changed block on the sides its
  * changes reach (the reach is below BLK, so only adjacent blocks), diagonal
- * ones when both of their sides are reached; under WRAP every block within
- * two, because a partial block at the seam is narrower than the reach. Each
- * joins the dirty list once (tagged with the pass); the sides are reset for
- * the next pass. */
+ * ones when both of their sides are reached. Within the S' reach a block
+ * recomputes S'; only within the detection reach, it re-emits its seeds from
+ * the S' it keeps (RESEED entries). Under WRAP every block within two
+ * recomputes, because a partial block at the seam is narrower than the reach.
+ * Each block joins the list once per kind (tagged with the pass); the sides
+ * are reset for the next pass. */
 __global__ __launch_bounds__(BLOCK) void
 expand_dirty(Params p, int iter)
 {
@@ -474,17 +479,23 @@ expand_dirty(Params p, int iter)
 				const int dx = k % side - r, dy = k / side - r;
 				int qx = bx + dx, qy = by + dy;
 				bool fresh = false;
-				const bool reached = wrap || ((dx == 0 || (sides & (dx < 0 ? SIDE_L : SIDE_R))) &&
-							      (dy == 0 || (sides & (dy < 0 ? SIDE_U : SIDE_D))));
+				auto reaches = [&](unsigned sd) {
+					return (dx == 0 || (sd & (dx < 0 ? SIDE_L : SIDE_R))) &&
+					       (dy == 0 || (sd & (dy < 0 ? SIDE_U : SIDE_D)));
+				};
+				const bool near_s = wrap || reaches(sides >> SIDES_S);
+				const bool reached = near_s || reaches(sides & 15u);
 				if (wrap) {
 					qx = (qx % nbx + nbx) % nbx;
 					qy = (qy % nby + nby) % nby;
 				}
 				const int q = block0 + qy * nbx + qx;
+				unsigned int *tags = near_s ? p.dirty_tag : p.reseed_tag;
 				if (e < n && reached && qx >= 0 && qx < nbx && qy >= 0 && qy < nby)
-					fresh = __atomic_load_n(&p.dirty_tag[q], __ATOMIC_RELAXED) != tag &&
-						atomicExch(&p.dirty_tag[q], tag) != tag;
-				lds_append(fresh, list_entry(f, q), out.item, &out.n);
+					fresh = __atomic_load_n(&tags[q], __ATOMIC_RELAXED) != tag &&
+						atomicExch(&tags[q], tag) != tag;
+				lds_append(fresh, list_entry(f, int(unsigned(q) | (near_s ? 0u : RESEED))), out.item,
+					&out.n);
 			}
 			block_list_flush(out, p.dirty_list, count);
 		}
@@ -1692,6 +1703,20 @@ significance_at(const Params &p, const Frame &fr, int x, int y, bool &bad)
 	return laplacian_q<EXACT, TAME>(qc, qu, qd, ql, qr, w == 1) / (2 * nz);
 }
 
+/* S' at pixel (x, y) of a frame if the last pass that processed the pixel
+ * computed it there (0 at holes, src/filters.cuh:137-146), else -inf: neither
+ * can pass a threshold t >= 0. A block that no block pass has processed is the
+ * first pass's (SCR); otherwise the pixel's stamp must be the block's. */
+__device__ __forceinline__ double
+sprime_at(const Params &p, long long off, int block0, int blocks_x, int x, int y, int w)
+{
+	long long g = off + (long long)y * w + x;
+	uint32_t fw = p.flags[g];
+	unsigned last = p.block_pass[block0 + (y / BLK) * blocks_x + x / BLK];
+	bool current = last ? p.stamp[g] == last : (fw & SCR) != 0;
+	return current && !(fw & HOLE) ? p.sprime[g] : -INFINITY;
+}
+
 constexpr int BR = BLK + 2 * HS; /* edge of the S region of a block */
 constexpr int WAVE_SEEDS = 256;  /* seeds buffered per wave between flushes */
 static_assert(BLK * BLK == 64, "one lane per pixel of a block");
@@ -1757,10 +1782,26 @@ significance_blocks_pass(BlockShared &sh, const Params &p, int iter)
 		const unsigned long long entry = uniform_u64(next);
 		if (e + waves < n)
 			next = p.dirty_list[e + waves];
-		const int f = (int)(entry >> 32), b = (int)(entry & 0xffffffffu);
+		const int f = (int)(entry >> 32), b = (int)((unsigned)entry & ~RESEED);
 		const Frame fr = p.frames[f];
 		const int lb = b - fr.block0, w = fr.w, h = fr.h;
 		const int X0 = lb % fr.blocks_x * BLK, Y0 = lb / fr.blocks_x * BLK;
+		if ((unsigned)entry & RESEED) {
+			/* S' is still current here: re-emit the seeds, unless the block
+			 * also recomputes in this pass */
+			if (__atomic_load_n(&p.dirty_tag[b], __ATOMIC_RELAXED) == stamp)
+				continue;
+			const int x = X0 + lane % BLK, y = Y0 + lane / BLK;
+			const bool seed = x < w && y < h &&
+				sprime_at(p, fr.off, fr.block0, fr.blocks_x, x, y, w) > t_cr;
+			lds_append(seed, list_entry(f, y * w + x), buf, nseed);
+			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+			__builtin_amdgcn_wave_barrier();
+			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+			if (*nseed > WAVE_SEEDS - 64)
+				flush_wave_seeds(buf, nseed, p.list_b, seeds);
+			continue;
+		}
 		if (!lane) {
 			p.block_pass[b] = stamp;
 			if (iter)
@@ -2029,20 +2070,6 @@ fine(Params p, int iter)
 			block_list_flush(out, p.list_c, count);
 	}
 	block_list_flush(out, p.list_c, count);
-}
-
-/* S' at pixel (x, y) of a frame if the last pass that processed the pixel
- * computed it there (0 at holes, src/filters.cuh:137-146), else -inf: neither
- * can pass a threshold t >= 0. A block that no block pass has processed is the
- * first pass's (SCR); otherwise the pixel's stamp must be the block's. */
-__device__ __forceinline__ double
-sprime_at(const Params &p, long long off, int block0, int blocks_x, int x, int y, int w)
-{
-	long long g = off + (long long)y * w + x;
-	uint32_t fw = p.flags[g];
-	unsigned last = p.block_pass[block0 + (y / BLK) * blocks_x + x / BLK];
-	bool current = last ? p.stamp[g] == last : (fw & SCR) != 0;
-	return current && !(fw & HOLE) ? p.sprime[g] : -INFINITY;
 }
 
 /* First growth (src/filters.cuh:177-193 at t): every pixel next to a
