@@ -1592,7 +1592,9 @@ significance_pass(SigShared &sh, const Params &p, int iter)
 			lds_barrier();
 
 			/* the deferred C of this wave's rows: its latency overlaps the
-			 * medians below */
+			 * medians below. Non-temporal: C is read again only by later
+			 * kernels, so it should not displace the input lines that
+			 * neighbouring tiles read from L2. */
 			if (deferred) {
 				const int ya = yb + r0;
 				double *cp = p.clean + fr.off + (long long)ya * w + X0 - HS + lane;
@@ -1600,7 +1602,7 @@ significance_pass(SigShared &sh, const Params &p, int iter)
 #pragma unroll
 				for (int r = 0; r < STEP_ROWS / 4; ++r)
 					if (r < nr && own_lane && ya + r >= Y0 && ya + r < own1)
-						cp[(long long)r * w] = cst[r];
+						__builtin_nontemporal_store(cst[r], &cp[(long long)r * w]);
 			}
 
 			/* 4. S' at the screened pixels, which get SCR (no hole lies in the
