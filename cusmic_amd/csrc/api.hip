@@ -251,7 +251,7 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	size_t n_shard = nf * size_t(SHARDS * SHARD_FIELDS);
 	size_t n_todo = size_t(std::max(maxiter, 1)) * nf;
 	size_t n_queue = size_t(std::max(maxiter, 1));
-	size_t n_nlist = 4 + 9 * size_t(std::max(maxiter, 1));
+	size_t n_nlist = 4 + 10 * size_t(std::max(maxiter, 1));
 	size_t words = n_counts + n_shard + n_todo + n_queue + n_nlist + 2 + 5 * size_t(nblocks) + size_t(ntiles);
 	size_t off_frames = 0;
 	size_t off_words = align_up(nf * sizeof(Frame), 256);
@@ -268,7 +268,12 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	size_t off_list_b = align_up(off_list + planes * 8, 256);
 	size_t off_list_c = align_up(off_list_b + planes * 8, 256);
 	size_t off_wide = align_up(off_list_c + planes * 8, 256);
-	size_t bytes = off_wide + planes * 8;
+	/* pixels within four of a frame edge: the seeds fine takes apart */
+	size_t edge = 0;
+	for (const auto &fr : frames)
+		edge += size_t(fr.w) * fr.h - size_t(std::max(fr.w - 8, 0)) * std::max(fr.h - 8, 0);
+	size_t off_edge = align_up(off_wide + planes * 8, 256);
+	size_t bytes = off_edge + (maxiter ? edge : 0) * 8;
 	scratch mem(bytes, stream);
 	char *base = static_cast<char *>(mem.ptr);
 	hip_check(hipMemcpyAsync(base + off_frames, frames.data(), nf * sizeof(Frame),
@@ -316,6 +321,7 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	p.list_b = reinterpret_cast<unsigned long long *>(base + off_list_b);
 	p.list_c = reinterpret_cast<unsigned long long *>(base + off_list_c);
 	p.wide_all = reinterpret_cast<unsigned long long *>(base + off_wide);
+	p.edge_list = reinterpret_cast<unsigned long long *>(base + off_edge);
 
 	unsigned table_grid = unsigned(std::min<long long>(((long long)nchunks + nruns + 255) / 256, 4096));
 	hipLaunchKernelGGL(set_tables, dim3(table_grid), dim3(256), 0, stream, p);
@@ -381,7 +387,8 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 			hipLaunchKernelGGL(expand_dirty, dim3(unsigned(cus) * 2), dim3(BLOCK), 0, stream, p, it);
 		hipLaunchKernelGGL(blocks, dim3(blocks_grid), dim3(BLOCK), 0, stream, p, it);
 		tm.mark(CUSMIC_PROFILE_SIGNIFICANCE_MS);
-		hipLaunchKernelGGL(fine, dim3(list_grid), dim3(BLOCK), 0, stream, p, it);
+		hipLaunchKernelGGL(fine<true>, dim3(list_grid), dim3(BLOCK), 0, stream, p, it);
+		hipLaunchKernelGGL(fine<false>, dim3(list_grid), dim3(BLOCK), 0, stream, p, it);
 		hipLaunchKernelGGL(grow_first, dim3(list_grid), dim3(BLOCK), 0, stream, p, it);
 		hipLaunchKernelGGL(grow_second, dim3(list_grid), dim3(BLOCK), 0, stream, p, it);
 		hipLaunchKernelGGL(fold_counts, dim3(fold_grid), dim3(256), 0, stream, p, it, 0);

@@ -135,6 +135,8 @@ struct Params {
 	unsigned int *dirty_tag;         /* [nblocks] stamp of the last pass that
 					  * listed the block as dirty */
 	unsigned long long *changed_list; /* frame << 32 | block, per pass */
+	unsigned long long *edge_list;   /* frame << 32 | pixel of the seeds of a
+					  * pass near a frame edge (not fine_inner) */
 	unsigned long long *dirty_list;  /* frame << 32 | block of the blocks the
 					  * block pass processes */
 	int *status;                     /* [0] error bits, [1] nonzero if a clean
@@ -180,6 +182,7 @@ __device__ __forceinline__ int nl_start(const Params &p, int it) { return 3 + 6 
 __device__ __forceinline__ int nl_redo(const Params &p, int it) { return 3 + 7 * p.maxiter + it; }
 __device__ __forceinline__ int nl_wide_base(const Params &p, int it) { return 3 + 8 * p.maxiter + it; }
 __device__ __forceinline__ int nl_wide_all(const Params &p) { return 3 + 9 * p.maxiter; }
+__device__ __forceinline__ int nl_edge(const Params &p, int it) { return 4 + 9 * p.maxiter + it; }
 
 /* scipy.ndimage border mapping of coordinate x on an axis of n samples;
  * -1 means a CONSTANT-mode zero (src/filters.cuh:24-44). */

@@ -1947,43 +1947,62 @@ median49_feed(int j, const double *acc, const double *row, double *out)
 	}
 }
 
+/* The 9 x 9 block of C samples around (x, y) lies inside the frame. */
+__device__ __forceinline__ bool
+fine_inner(int x, int y, int w, int h)
+{
+	return x >= 4 && x + 4 < w && y >= 4 && y + 4 < h;
+}
+
 /* F at a seed (src/filters.cuh:148-162): m3 = med3 of the clean image (median
  * border mode) over the 7 x 7 positions around it, med7 = their median, fed row
- * by row so that at most 22 ranks stay live, and N at the seed. */
+ * by row so that at most 22 ranks stay live, and N at the seed. INNER: the seed
+ * is fine_inner, and the code has no border paths (about 220 VGPRs, two waves
+ * per SIMD, where the border paths need more than 256). */
+template <bool INNER>
 __device__ double
 fine_structure(const Params &p, const Frame &fr, int x, int y)
 {
 	const double *clean = p.clean + fr.off;
 	const int w = fr.w, h = fr.h, mode = p.mode;
-	const bool inside = x >= 3 && x + 3 < w && y >= 3 && y + 3 < h;
+	const bool inside = INNER || (x >= 3 && x + 3 < w && y >= 3 && y + 3 < h);
 	/* in the extended image of a symmetric or periodic mode, med3 at an outside
 	 * position equals med3 at its mapped position; CONSTANT positions outside
 	 * the frame are 0; NEAREST near the edge takes the clamped position */
 	const bool block = inside || mode != CUSMIC_NEAREST;
-	/* the 9 x 9 block of C lies inside the frame: one row pointer */
-	const bool direct = x >= 4 && x + 4 < w && y >= 4 && y + 4 < h;
-	const double *rp = clean + (long long)(y - 4) * w + (x - 4);
+	/* the 9 x 9 block of C samples: INNER reads it through one row pointer;
+	 * otherwise its columns and each row are mapped once (border_index, -1 for
+	 * a CONSTANT sample outside, which is 0) */
+	int bx[9];
+#pragma unroll
+	for (int i = 0; i < 9; ++i)
+		bx[i] = INNER ? x - 4 + i : border_index(x - 4 + i, w, mode);
+	auto load_row = [&](int yy, double *dst) {
+		if (INNER) {
+			const double *r = clean + (long long)yy * w;
+#pragma unroll
+			for (int i = 0; i < 9; ++i)
+				dst[i] = r[bx[i]];
+		} else {
+			const int by = border_index(yy, h, mode);
+			const double *r = clean + (long long)max(by, 0) * w;
+#pragma unroll
+			for (int i = 0; i < 9; ++i) {
+				const double v = r[max(bx[i], 0)];
+				dst[i] = by < 0 || bx[i] < 0 ? 0.0 : v;
+			}
+		}
+	};
 	double acc[2][22], a[9], b[9], c[9], centre = 0.0;
 	if (block) {
-#pragma unroll
-		for (int i = 0; i < 9; ++i) {
-			a[i] = direct ? rp[i] : med_sample(clean, x - 4 + i, y - 4, w, h, mode);
-			b[i] = direct ? rp[w + i] : med_sample(clean, x - 4 + i, y - 3, w, h, mode);
-		}
+		load_row(y - 4, a);
+		load_row(y - 3, b);
 	}
 #pragma unroll
 	for (int j = 0; j < 7; ++j) {
 		double row[7];
 		if (block) {
-			if (direct) {
-#pragma unroll
-				for (int i = 0; i < 9; ++i)
-					c[i] = rp[(long long)(j + 2) * w + i];
-			} else {
-#pragma unroll
-				for (int i = 0; i < 9; ++i)
-					c[i] = med_sample(clean, x - 4 + i, y - 2 + j, w, h, mode);
-			}
+			load_row(y - 2 + j, c);
 			/* 3x3 medians from sorted columns: med9 = med3(max of minima,
 			 * med3 of middles, min of maxima) */
 			double lo[9], md[9], hi[9];
@@ -2032,9 +2051,12 @@ fine_structure(const Params &p, const Frame &fr, int x, int y)
 		nz = fr.error[y * w + x];
 	} else {
 		double v25[25], m5;
-		for (int j = 0; j < 5; ++j)
+		for (int j = 0; j < 5; ++j) {
+			double r9[9];
+			load_row(y - 2 + j, r9);
 			for (int i = 0; i < 5; ++i)
-				v25[j * 5 + i] = med_sample(clean, x - 2 + i, y - 2 + j, w, h, mode);
+				v25[j * 5 + i] = r9[i + 2];
+		}
 		median25(v25, &m5);
 		double g = fr.gain ? fr.gain[y * w + x] : fr.g;
 		double rn = fr.readnoise ? fr.readnoise[y * w + x] : fr.rn;
@@ -2044,27 +2066,39 @@ fine_structure(const Params &p, const Frame &fr, int x, int y)
 	return fine < FINE_FLOOR ? FINE_FLOOR : fine;
 }
 
+/* Candidates among the seeds of pass iter: S' / F > contrast. The INNER
+ * launch reads the seeds, takes the fine_inner ones and lists the others not
+ * excluded (edge_list); the second launch takes those. */
+template <bool INNER>
 __global__ __launch_bounds__(BLOCK) void
 fine(Params p, int iter)
 {
 	__shared__ BlockList<2 * BLOCK> out; /* flushed when the next round might not fit */
-	const unsigned n = p.nlist[nl_seed(p, iter)];
+	const unsigned n = p.nlist[INNER ? nl_seed(p, iter) : nl_edge(p, iter)];
+	const unsigned long long *seeds = INNER ? p.list_b : p.edge_list;
 	unsigned int *count = &p.nlist[nl_cand(p, iter)];
 	if (!threadIdx.x)
 		out.n = 0;
 	for (unsigned base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
 		unsigned e = base + threadIdx.x;
-		bool cand = false;
+		bool cand = false, edge = false;
 		unsigned long long entry = 0;
 		if (e < n) {
-			entry = p.list_b[e];
+			entry = seeds[e];
 			int f = (int)(entry >> 32), i = (int)(entry & 0xffffffffu);
 			const Frame fr = p.frames[f];
+			const int x = i % fr.w, y = i / fr.w;
 			if (!(p.flags[fr.off + i] & EXCL)) {
-				double sp = p.sprime[fr.off + i];
-				cand = sp / fine_structure(p, fr, i % fr.w, i / fr.w) > p.contrast;
+				if (INNER && !fine_inner(x, y, fr.w, fr.h)) {
+					edge = true;
+				} else {
+					double sp = p.sprime[fr.off + i];
+					cand = sp / fine_structure<INNER>(p, fr, x, y) > p.contrast;
+				}
 			}
 		}
+		if (INNER)
+			list_push(edge, entry, p.edge_list, &p.nlist[nl_edge(p, iter)]);
 		__syncthreads();
 		lds_append(cand, entry, out.item, &out.n);
 		__syncthreads();
