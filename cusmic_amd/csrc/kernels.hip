@@ -1324,18 +1324,32 @@ significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0
 		gdouble *dp = (gdouble *)(fr.data + o), *ep = (gdouble *)(fr.error + o);
 		gdouble *bp = BG && fr.background ? (gdouble *)(fr.background + o) : nullptr;
 		/* the left and right samples of a row are the neighbouring lanes'
-		 * centres (a wave shift); lanes 0 and 63 load theirs, the others
-		 * reload their centre in the same instruction */
+		 * centres (a wave shift). Lanes 0 and 63 take theirs from two scalar
+		 * loads per row (the address is the wave's; the input is read-only
+		 * here), which leaves one vector load fewer per row and the registers
+		 * for a third row in flight. This is the error-map kernel without a
+		 * background (SEDGE); elsewhere the edge samples are vector loads:
+		 * lanes 0 and 63 load theirs, the others reload their centre in the
+		 * same instruction, two rows ahead. */
+		constexpr bool SEDGE = !MODEL && !BG;
 		const int eo = lane == 0 ? -1 : lane == 63 ? 1 : 0;
-		/* the samples of row r: the pixel below, the edge sample and the
+		using cdouble = const __attribute__((address_space(4))) double;
+		const unsigned long long lb = uniform_u64((unsigned long long)(fr.data + (long long)ya * w + (X0 - HS - 1)));
+		cdouble *dl = (cdouble *)lb, *dr = (cdouble *)(lb + 8ull * (SW + 1));
+		/* the samples of row r: the pixel below, the edge samples and the
 		 * noise (and their background), fetched PF rows ahead */
-		constexpr int PF = 2;
-		double dn[NR], de[NR], nz[NR], bn[NR], be[NR];
+		constexpr int PF = SEDGE ? 3 : 2;
+		double dn[NR], de[NR], el[NR], er[NR], nz[NR], bn[NR], be[NR];
 #define FETCH_ROW(k)                                                                       \
 	do {                                                                               \
 		const long long ro = (long long)(k) * W;                                   \
 		dn[k] = dp[ro + W];                                                        \
-		de[k] = dp[ro + eo];                                                       \
+		if (!SEDGE) {                                                              \
+			de[k] = dp[ro + eo];                                               \
+		} else {                                                                   \
+			el[k] = dl[ro];                                                    \
+			er[k] = dr[ro];                                                    \
+		}                                                                          \
 		nz[k] = ep[ro];                                                            \
 		bn[k] = be[k] = p.bg;                                                      \
 		if (BG && bp) {                                                            \
@@ -1359,7 +1373,10 @@ significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0
 				FETCH_ROW(r + PF);
 			bool hd = false, hs = false;
 			const double cd = input_c<BG>(dn[r], bn[r], has_bg, hd);
-			const double qe = input_c<BG>(de[r], be[r], has_bg, hs) * 0.25;
+			const double qe = input_c<BG>(SEDGE ? lane == 0 ? el[r] : er[r] : de[r], be[r], has_bg, hs) * 0.25;
+			/* the other lanes' hole flags come from their own samples */
+			if (SEDGE)
+				hs = hs && (lane == 0 || lane == 63);
 			const double ql = wave_shift_right(qc, qe), qr = wave_shift_left(qc, qe);
 			const double qd = cd * 0.25;
 			const int gy = ya + r;
