@@ -1740,8 +1740,11 @@ constexpr int BR = BLK + 2 * HS; /* edge of the S region of a block */
 constexpr int WAVE_SEEDS = 256;  /* seeds buffered per wave between flushes */
 static_assert(BLK * BLK == 64, "one lane per pixel of a block");
 
+constexpr int CR = BR + 2; /* edge of the C region of a block: S and its ring */
+
 struct BlockShared {
 	double s[BLOCK / 64][BR * BR];
+	double c[BLOCK / 64][CR * CR];
 	unsigned long long seeds[BLOCK / 64][WAVE_SEEDS];
 	int nseed[BLOCK / 64];
 	unsigned base;
@@ -1829,7 +1832,8 @@ significance_blocks_pass(BlockShared &sh, const Params &p, int iter)
 		bool bad = false;
 		/* the region and its one-pixel ring lie inside the frame */
 		const bool inner = X0 - HS - 1 >= 0 && X0 + BLK + HS < w && Y0 - HS - 1 >= 0 && Y0 + BLK + HS < h;
-		if ((!MODEL || fr.error) && inner) {
+		if (MODEL && fr.error && inner) {
+			/* (the model kernel keeps its registers: five loads per S) */
 			const double *cb = p.clean + fr.off + (long long)(Y0 - HS) * w + (X0 - HS);
 			const double *eb = fr.error + (long long)(Y0 - HS) * w + (X0 - HS);
 #pragma unroll
@@ -1842,6 +1846,38 @@ significance_blocks_pass(BlockShared &sh, const Params &p, int iter)
 						cb[o + w] * 0.25, cb[o - 1] * 0.25, cb[o + 1] * 0.25, false);
 					bad |= invalid_noise(nz);
 					S[k] = q / (2 * nz);
+				}
+			}
+		} else if (!MODEL && inner) {
+			/* the C region (S and its ring) once into the wave's LDS, then
+			 * the stencil from there */
+			const double *cb = p.clean + fr.off + (long long)(Y0 - HS - 1) * w + (X0 - HS - 1);
+			const double *eb = fr.error + (long long)(Y0 - HS) * w + (X0 - HS);
+			double *Ct = sh.c[wave];
+			double nz[3];
+#pragma unroll
+			for (int j = 0; j < 3; ++j) {
+				const int k = lane + 64 * j;
+				nz[j] = j < 2 || k < BR * BR ? eb[(k / BR) * w + k % BR] : 1.0;
+			}
+#pragma unroll
+			for (int j = 0; j < 4; ++j) {
+				const int k = lane + 64 * j;
+				if (j < 3 || k < CR * CR)
+					Ct[k] = cb[(k / CR) * w + k % CR];
+			}
+			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+			__builtin_amdgcn_wave_barrier();
+			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+			for (int j = 0; j < 3; ++j) {
+				const int k = lane + 64 * j;
+				if (j < 2 || k < BR * BR) {
+					const int c = (k / BR + 1) * CR + k % BR + 1;
+					double q = laplacian_q<EXACT, TAME>(Ct[c] * 0.25, Ct[c - CR] * 0.25,
+						Ct[c + CR] * 0.25, Ct[c - 1] * 0.25, Ct[c + 1] * 0.25, false);
+					bad |= invalid_noise(nz[j]);
+					S[k] = q / (2 * nz[j]);
 				}
 			}
 		} else {
