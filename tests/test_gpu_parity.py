@@ -232,6 +232,25 @@ def test_tiny_and_odd_shapes(cuda):
             compare([dict(data=d, error=np.full(shape, 2.0))], border_mode=mode)
 
 
+@pytest.mark.parametrize("mode", MODES)
+def test_rays_on_frame_edges(cuda, mode):
+    """Cosmic-ray pixels on the four edges, at the corners and one to four
+    pixels in: their seeds take the fine-structure path for windows that cross
+    the frame edge (the border mode's extension), over several passes."""
+    rng = np.random.default_rng(11)
+    h, w = 96, 80
+    d = rng.normal(100, 3, size=(h, w))
+    hits = [(0, 0), (0, w - 1), (h - 1, 0), (h - 1, w - 1), (0, 37), (h - 1, 21), (45, 0),
+            (61, w - 1), (1, 1), (2, w - 3), (h - 3, 2), (3, 40), (50, 3), (h - 4, w - 4), (4, 4)]
+    for y, x in hits:
+        d[y, x] += 1500
+        d[min(y + 1, h - 1), x] += 900
+    say("edge rays", mode)
+    compare([dict(data=d, error=np.full((h, w), 3.0))], border_mode=mode, maxiter=6,
+            record_donors=True)
+    compare([dict(data=d)], border_mode=mode, frame_gain=[2.0], frame_readnoise=[5.0], maxiter=6)
+
+
 def test_ragged_batch_equals_frames(cuda):
     shapes = ((64, 64), (31, 130), (200, 77), (1, 50), (96, 96))
     frames = []
