@@ -14,7 +14,8 @@ resident in HBM. A step is one such call. Also reported:
     duration; call_roofline: the whole call's algorithmic bytes
     (17 + M + I_d (8 + E) + I_r 8 per pixel) over the step time;
   * cpu_baseline: the oracle (the C restatement, one thread) on a bounded
-    sample of the same workload.
+    sample of the same workload; cpu_baseline_all_cores: the same oracle on
+    every host core (OpenMP over rows) on up to four 1024^2 frames.
 Prints one JSON line on stdout; progress goes to stderr.
 """
 
@@ -316,6 +317,22 @@ def main():
                                   "sample": f"{len(frames)} frame(s) of {shape} from the same ragged "
                                             f"batch (seed 3), error map, maxiter 4; {c_s:.1f} s"}
         log(f"cpu: {spx / c_s:.3f} Mpix/s")
+        # the same oracle on every host core (OpenMP over rows), on up to four
+        # 1024^2 frames of the batch
+        wide = [b for b in batch if b["data"].numel() == 1 << 20][:4] or batch[:1]
+        wframes = [dict(data=b["data"].cpu().numpy(), error=b["error"].cpu().numpy()) for b in wide]
+        wpx = sum(f["data"].size for f in wframes) / 1e6
+        log(f"cpu baseline: oracle, {cores} threads, {wpx:.2f} Mpx")
+        t0 = time.perf_counter()
+        oracle.clean_frames(wframes, threads=cores)
+        a_s = time.perf_counter() - t0
+        result["cpu_baseline_all_cores"] = {
+            "value": round(wpx / a_s, 4), "unit": "Mpix/s", "cores": cores, "kind": "port",
+            "cpu": model,
+            "sample": f"{len(wframes)} frame(s) of "
+                      f"{'x'.join(map(str, wframes[0]['data'].shape))} from the same ragged batch, "
+                      f"error map, maxiter 4, OpenMP over rows; {a_s:.1f} s"}
+        log(f"cpu, {cores} threads: {wpx / a_s:.3f} Mpix/s")
     print(json.dumps(result), flush=True)
 
 
