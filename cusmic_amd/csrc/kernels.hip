@@ -985,7 +985,7 @@ redo_scan(Params p, int iter)
 /* ------------------------------------------------------------------------ */
 /* Detection (src/filters.cuh:86-193, src/cleaner.cuh:22-37,102-126)         */
 /*
- * One detection pass is four launches:
+ * One detection pass is four stages:
  *   significance  S = L / (2 N) over a region and a 2-pixel halo in LDS;
  *                 S' = S - med5(S) only where S > min(t, t_n) (S' <= S because
  *                 S >= 0), written to a sparse plane; seeds (S' > t) go to a
@@ -997,7 +997,9 @@ redo_scan(Params p, int iter)
  *                 tiles whose S waits for the hole fill, later those near a
  *                 pixel the last replacement changed.
  *   fine          per seed: F = max((m3 - med7(m3)) / N, 0.01), m3 = med3(C);
- *                 S' / F > contrast puts the seed on the candidate list
+ *                 S' / F > contrast puts the seed on the candidate list (two
+ *                 launches: seeds whose 9 x 9 samples lie inside the frame,
+ *                 then those near an edge)
  *   grow_first    per candidate: neighbours with S' > t join the growth list
  *   grow_second   per grown pixel: neighbours with S' > t_n are flagged; a
  *                 pixel flagged for the first time is new (accumulate)
