@@ -15,7 +15,7 @@ resident in HBM. A step is one such call. Also reported:
     (17 + M + I_d (8 + E) + I_r 8 per pixel) over the step time;
   * cpu_baseline: the oracle (the C restatement, one thread) on a bounded
     sample of the same workload; cpu_baseline_all_cores: the same oracle on
-    every host core (OpenMP over rows) on up to four 1024^2 frames.
+    every host core (OpenMP over rows) on up to four 1 Mpx frames.
 Prints one JSON line on stdout; progress goes to stderr.
 """
 
@@ -318,7 +318,7 @@ def main():
                                             f"batch (seed 3), error map, maxiter 4; {c_s:.1f} s"}
         log(f"cpu: {spx / c_s:.3f} Mpix/s")
         # the same oracle on every host core (OpenMP over rows), on up to four
-        # 1024^2 frames of the batch
+        # 1 Mpx frames of the batch
         wide = [b for b in batch if b["data"].numel() == 1 << 20][:4] or batch[:1]
         wframes = [dict(data=b["data"].cpu().numpy(), error=b["error"].cpu().numpy()) for b in wide]
         wpx = sum(f["data"].size for f in wframes) / 1e6
