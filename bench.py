@@ -85,9 +85,8 @@ def main():
                          "readings scale FETCH_SIZE/WRITE_SIZE to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: every GPU box in this setup has one GPU, and the "
-              "multi-GPU design (frames sharded by pixel count, no exchange) is described in "
-              "DESIGN.md", file=sys.stderr)
+        print("bench.py measures one GPU only; on several GPUs the frames of a batch would be "
+              "sharded by pixel count, with no exchange between GPUs", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -103,7 +102,7 @@ def main():
         torch.cuda.synchronize()
         del a
     log("generating the ragged batch on the GPU")
-    batch = scenes.config3(dev, n=args.frames)
+    batch = scenes.ragged_batch(dev, n=args.frames)
     fr = engine.Frames()
     for item in batch:
         fr.add(item["data"], error=item["error"])
@@ -266,8 +265,8 @@ def main():
         log("forward+backward, 8 x 1024^2 noise-model frames")
         from cusmic_amd import autograd
 
-        c4 = scenes.config4(dev)
-        data, w = c4["data"], c4["weights"]
+        adj = scenes.adjoint_batch(dev)
+        data, w = adj["data"], adj["weights"]
         mp4 = data.numel() / 1e6
 
         def fwd():

@@ -88,13 +88,13 @@ def frame(shape, seed, device, sky=1000.0, n_stars=40, n_tracks=150, max_len=4, 
     return data, signal
 
 
-def config0(device, seed=0):
+def single_frame_512(device, seed=0):
     """One 512x512 frame, noise-model path (gain 2, read noise 5)."""
     data, _ = frame((512, 512), seed, device)
     return dict(data=data, gain=2.0, readnoise=5.0)
 
 
-def config1(device, n=16, seed=1, shape=(512, 512)):
+def stack_512(device, n=16, seed=1, shape=(512, 512)):
     """Stack of n frames with a shared error map, 2 bad columns per frame in the
     mask and 0.01% NaN pixels."""
     g = _gen(seed + 1000, device)
@@ -114,7 +114,7 @@ def config1(device, n=16, seed=1, shape=(512, 512)):
     return dict(data=data, error=error_map(signal, 2.0, 5.0), mask=mask)
 
 
-def config2(device, seed=2, shape=(4096, 4096)):
+def detector_frame(device, seed=2, shape=(4096, 4096)):
     """Detector-size frame: sky gradient 200..3000, 2000 stars, 8000 tracks of
     1..12 px, per-pixel error map, scalar background 150, 0.5% masked."""
     h, w = shape
@@ -131,7 +131,7 @@ def config2(device, seed=2, shape=(4096, 4096)):
 RAGGED_SHAPES = ((256, 256), (512, 2048), (1024, 1024), (2048, 4096), (4096, 4096))
 
 
-def config3(device, n=64, seed=3, shapes=RAGGED_SHAPES, density=2e-4):
+def ragged_batch(device, n=64, seed=3, shapes=RAGGED_SHAPES, density=2e-4):
     """Ragged batch: shapes drawn uniformly from ``shapes``, per-frame seeds,
     per-frame error maps, 2e-4 tracks/px, gain in [1,4], read noise in [2,10]."""
     g = _gen(seed, "cpu")
@@ -151,8 +151,9 @@ def config3(device, n=64, seed=3, shapes=RAGGED_SHAPES, density=2e-4):
     return out
 
 
-def config4(device, n=8, seed=4, shape=(1024, 1024)):
-    """Batch of n frames from the config-0 generator plus seeded weights."""
+def adjoint_batch(device, n=8, seed=4, shape=(1024, 1024)):
+    """Batch of n noise-model frames (the single-frame generator) plus seeded
+    weights."""
     frames = [frame(shape, seed + 10 * f, device)[0] for f in range(n)]
     g = _gen(seed + 999, device)
     data = torch.stack(frames)

@@ -1,6 +1,6 @@
 """Adjoint of the cleaning map with the mask held fixed (GPU).
 
-Checks, on noise-model frames from the config-0 generator:
+Checks, on noise-model frames from the single-frame generator:
   * the recorded donors equal the oracle's;
   * grad_data equals a CPU gradient built from the oracle's donor choices,
     summed in the same order (bit for bit);
