@@ -943,7 +943,7 @@ replace_widened(Params p, int iter, int fill)
 __global__ __launch_bounds__(BLOCK) void
 redo_scan(Params p, int iter)
 {
-	__shared__ BlockList<BLOCK * 8> out;
+	__shared__ BlockList<BLOCK * 24> out;
 	const unsigned start = p.nlist[nl_start(p, iter)], n = p.nlist[0] - start;
 	unsigned int *count = &p.nlist[nl_redo(p, iter)];
 	if (!threadIdx.x)
@@ -963,22 +963,30 @@ redo_scan(Params p, int iter)
 			x = i % w;
 			y = i / w;
 		}
-		/* 24 neighbours in rounds of 8, one buffer flush each */
-		for (int k0 = 0; k0 < 24; k0 += 8) {
-			__syncthreads();
-			for (int k = k0; k < k0 + 8; ++k) {
-				const int o = k < 12 ? k : k + 1; /* skip the centre */
-				const int qx = x + o % 5 - 2, qy = y + o / 5 - 2, q = qy * w + qx;
-				bool redo = false;
-				if (e < n && qx >= 0 && qx < w && qy >= 0 && qy < h) {
-					uint32_t fl = p.flags[off + q];
-					if ((fl & CRAY) && !(fl & (NEWP | WIDE | REDO)))
-						redo = !(flags_or(&p.flags[off + q], REDO) & REDO);
-				}
-				lds_append(redo, list_entry(f, q), out.item, &out.n);
-			}
-			block_list_flush(out, p.list_b, count);
+		/* the 24 neighbours' flags in one round trip, then the atomics of the
+		 * few to redo, then one buffer flush */
+		uint32_t fl[24];
+#pragma unroll
+		for (int k = 0; k < 24; ++k) {
+			const int o = k < 12 ? k : k + 1; /* skip the centre */
+			const int qx = x + o % 5 - 2, qy = y + o / 5 - 2;
+			fl[k] = e < n && qx >= 0 && qx < w && qy >= 0 && qy < h ? uint32_t(p.flags[off + qy * w + qx]) : 0u;
 		}
+		bool redo[24];
+#pragma unroll
+		for (int k = 0; k < 24; ++k) {
+			const int o = k < 12 ? k : k + 1;
+			const int q = (y + o / 5 - 2) * w + x + o % 5 - 2;
+			redo[k] = (fl[k] & CRAY) && !(fl[k] & (NEWP | WIDE | REDO)) &&
+				!(flags_or(&p.flags[off + q], REDO) & REDO);
+		}
+		__syncthreads();
+#pragma unroll
+		for (int k = 0; k < 24; ++k) {
+			const int o = k < 12 ? k : k + 1;
+			lds_append(redo[k], list_entry(f, (y + o / 5 - 2) * w + x + o % 5 - 2), out.item, &out.n);
+		}
+		block_list_flush(out, p.list_b, count);
 	}
 }
 
