@@ -391,9 +391,11 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 		hipLaunchKernelGGL(fine<false>, dim3(list_grid), dim3(BLOCK), 0, stream, p, it);
 		hipLaunchKernelGGL(grow_first, dim3(list_grid), dim3(BLOCK), 0, stream, p, it);
 		hipLaunchKernelGGL(grow_second, dim3(list_grid), dim3(BLOCK), 0, stream, p, it);
-		hipLaunchKernelGGL(fold_counts, dim3(fold_grid), dim3(256), 0, stream, p, it, 0);
+		/* the pass's counts; after the first pass redo_scan folds them */
 		if (it > 0)
 			hipLaunchKernelGGL(redo_scan, dim3(list_grid), dim3(BLOCK), 0, stream, p, it);
+		else
+			hipLaunchKernelGGL(fold_counts, dim3(fold_grid), dim3(256), 0, stream, p, it, 0);
 		tm.mark(CUSMIC_PROFILE_DETECT_MS);
 		hipLaunchKernelGGL(replace_pass, dim3(replace_grid), dim3(BLOCK), 0, stream, p, it, 0);
 		hipLaunchKernelGGL(replace_widened, dim3(cus), dim3(BLOCK), 0, stream, p, it, 0);

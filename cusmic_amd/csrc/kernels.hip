@@ -148,13 +148,15 @@ zero_groups(uint4 *groups, size_t n)
 /* Fold the pending per-frame count shards into the totals (and, after a
  * detection pass, the pass's new-pixel count) and clear them: SHARDS lanes
  * per frame, one shard each, summed across the lanes. */
-__global__ void
-fold_counts(Params p, int iter, int first)
+/* The fold of global thread t (t / SHARDS is the frame): every lane of the
+ * wave calls it, with the waves' t aligned to SHARDS. */
+__device__ __forceinline__ void
+fold_shards(const Params &p, int iter, int first, unsigned t)
 {
 	static_assert(SHARDS == 32, "a frame's shards fill half a wave");
-	const int t = blockIdx.x * blockDim.x + threadIdx.x, f = t / SHARDS;
+	const unsigned f = t / SHARDS;
 	int d[SHARD_FIELDS] = {};
-	if (f < p.nf) {
+	if (f < unsigned(p.nf)) {
 		int *sh = p.shard + (size_t)t * SHARD_FIELDS;
 #pragma unroll
 		for (int k = 0; k < SHARD_FIELDS; ++k) {
@@ -166,13 +168,19 @@ fold_counts(Params p, int iter, int first)
 	for (int k = 0; k < SHARD_FIELDS; ++k)
 		for (int o = SHARDS / 2; o; o >>= 1)
 			d[k] += __shfl_xor(d[k], o, 64);
-	if (f < p.nf && !(t % SHARDS)) {
+	if (f < unsigned(p.nf) && !(t % SHARDS)) {
 		p.counts[f].holes += d[SHARD_HOLES];
 		p.counts[f].allowed += d[SHARD_ALLOWED];
 		p.counts[f].donors += d[SHARD_DONORS];
 		if (!first)
 			p.todo[iter * p.nf + f] = d[SHARD_FRESH];
 	}
+}
+
+__global__ void
+fold_counts(Params p, int iter, int first)
+{
+	fold_shards(p, iter, first, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 /* Donor records start as (-1, -1): no donor. */
@@ -939,7 +947,8 @@ replace_widened(Params p, int iter, int fill)
 /* Older flagged pixels within 2 pixels of a pixel that pass iter flagged have
  * lost a donor from their 5x5 window: each joins the pass's redo list once
  * (REDO bit). Pixels flagged in this pass (NEWP) are replaced anyway and
- * pixels on wide_all (WIDE) are redone anyway. */
+ * pixels on wide_all (WIDE) are redone anyway. The launch also folds the
+ * pass's count shards, which saves a launch of fold_counts. */
 __global__ __launch_bounds__(BLOCK) void
 redo_scan(Params p, int iter)
 {
@@ -948,6 +957,10 @@ redo_scan(Params p, int iter)
 	unsigned int *count = &p.nlist[nl_redo(p, iter)];
 	if (!threadIdx.x)
 		out.n = 0;
+	/* the pass's counts first (fold_counts for passes after the first: nothing
+	 * here reads them, the replacement after this launch does) */
+	for (unsigned t0 = blockIdx.x * BLOCK; t0 < unsigned(p.nf) * SHARDS; t0 += gridDim.x * BLOCK)
+		fold_shards(p, iter, 0, t0 + threadIdx.x);
 	for (unsigned base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
 		const unsigned e = base + threadIdx.x;
 		int f = 0, x = 0, y = 0, w = 1, h = 1;
