@@ -253,12 +253,14 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	size_t n_queue = size_t(std::max(maxiter, 1));
 	size_t n_nlist = 4 + 10 * size_t(std::max(maxiter, 1));
 	size_t words = n_counts + n_shard + n_todo + n_queue + n_nlist + 2 + 5 * size_t(nblocks) + size_t(ntiles);
+	/* the per-pass pixel counts (64-bit) close the zeroed words */
+	const size_t work_word = align_up(words, 2);
+	words = work_word + 2 * n_queue;
 	size_t off_frames = 0;
 	size_t off_words = align_up(nf * sizeof(Frame), 256);
 	size_t off_dlist = align_up(off_words + words * 4, 256);
 	size_t off_clist = align_up(off_dlist + size_t(nblocks) * 16, 256); /* recompute + reseed */
-	size_t off_work = align_up(off_clist + size_t(nblocks) * 8, 256);
-	size_t off_cframe = align_up(off_work + n_queue * 8, 256);
+	size_t off_cframe = align_up(off_clist + size_t(nblocks) * 8, 256);
 	size_t off_rframe = align_up(off_cframe + size_t(nchunks) * 4, 256);
 	size_t off_flags = align_up(off_rframe + size_t(nruns) * 4, 256);
 	size_t planes = maxiter ? total : 0;
@@ -309,7 +311,7 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	p.changed_list = reinterpret_cast<unsigned long long *>(base + off_clist);
 	p.nblocks = nblocks;
 	p.dirty_list = reinterpret_cast<unsigned long long *>(base + off_dlist);
-	p.work = reinterpret_cast<unsigned long long *>(base + off_work);
+	p.work = reinterpret_cast<unsigned long long *>(w32 + work_word);
 	p.chunk_frame = reinterpret_cast<int *>(base + off_cframe);
 	p.run_frame = reinterpret_cast<int *>(base + off_rframe);
 	p.nruns = nruns;
@@ -345,8 +347,6 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	tm.mark(-1);
 	hipLaunchKernelGGL(zero_words, dim3(unsigned(std::min<size_t>((words + 255) / 256, 4096))),
 		dim3(256), 0, stream, w32, words);
-	hipLaunchKernelGGL(zero_words, dim3(1), dim3(256), 0, stream,
-		reinterpret_cast<unsigned int *>(p.work), n_queue * 2);
 	hipLaunchKernelGGL(zero_groups, dim3(unsigned(std::min<size_t>((total / 16 + 256) / 256, size_t(cus) * 8))),
 		dim3(256), 0, stream, reinterpret_cast<uint4 *>(p.flags), (total + 15) / 16);
 	if (donors)
