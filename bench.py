@@ -196,10 +196,21 @@ def main():
             per_frame()
         torch.cuda.synchronize()
         pf_ms = 1e3 * (time.perf_counter() - t0) / reps
+        # outside the timed region: every frame's plain-path output equals its
+        # slice of the ragged call's output, bit for bit
+        same, off = 0, 0
+        for f1 in singles:
+            n1 = f1.total
+            c1, m1, _, _ = engine.run(f1, dev, 3.0, 5.0, 3.0, 4, _lib.MIRROR)
+            same += bool(torch.equal(c1.view(torch.int64), clean[off:off + n1].view(torch.int64))
+                         and torch.equal(m1, mask[off:off + n1]))
+            off += n1
         result["per_frame"] = {"value": round(mpix / (pf_ms / 1e3), 2), "unit": "Mpix/s",
                                "ms_per_batch": round(pf_ms, 3),
-                               "ragged_speedup": round(pf_ms / ms_step, 3)}
-        log(f"per-frame: {pf_ms:.3f} ms/batch")
+                               "ragged_speedup": round(pf_ms / ms_step, 3),
+                               "outputs_equal_ragged": same == len(singles),
+                               "frames_equal": same}
+        log(f"per-frame: {pf_ms:.3f} ms/batch; {same}/{len(singles)} frames equal the ragged output")
         del singles
 
     if not args.no_host:
@@ -303,18 +314,30 @@ def main():
         import oracle
 
         cores, model = cpu_info()
-        sample = [b for b in batch if b["data"].numel() == 1 << 20][:1] or batch[:1]
+        # the first 1 Mpx frame of the batch, and where its pixels sit in the
+        # ragged call's output
+        offs = np.cumsum([0] + [b["data"].numel() for b in batch])
+        pick = next((k for k, b in enumerate(batch) if b["data"].numel() == 1 << 20), 0)
+        sample = [batch[pick]]
         frames = [dict(data=b["data"].cpu().numpy(), error=b["error"].cpu().numpy()) for b in sample]
         spx = sum(f["data"].size for f in frames) / 1e6
         log(f"cpu baseline: oracle, 1 thread, {spx:.2f} Mpx")
         t0 = time.perf_counter()
-        oracle.clean_frames(frames, threads=1)
+        o_clean, o_mask, _, _ = oracle.clean_frames(frames, threads=1)
         c_s = time.perf_counter() - t0
+        # outside the timed regions: the ragged call's frame equals the oracle's
+        lo, hi = int(offs[pick]), int(offs[pick + 1])
+        g_clean = clean[lo:hi].cpu().numpy().reshape(o_clean[0].shape)
+        g_mask = mask[lo:hi].cpu().numpy().reshape(o_mask[0].shape).astype(bool)
+        oracle_equal = bool((g_clean.view(np.uint64) == o_clean[0].view(np.uint64)).all()
+                            and (g_mask == o_mask[0]).all())
         shape = "x".join(map(str, frames[0]["data"].shape))
         result["cpu_baseline"] = {"value": round(spx / c_s, 4), "unit": "Mpix/s", "cores": 1,
                                   "kind": "port", "cpu": model, "host_cores": cores,
-                                  "sample": f"{len(frames)} frame(s) of {shape} from the same ragged "
-                                            f"batch (seed 3), error map, maxiter 4; {c_s:.1f} s"}
+                                  "sample": f"frame {pick} ({shape}) of the same ragged batch "
+                                            f"(seed 3), error map, maxiter 4; {c_s:.1f} s",
+                                  "gpu_output_equals_oracle": oracle_equal}
+        log(f"cpu: oracle equals the ragged call's frame {pick}: {oracle_equal}")
         log(f"cpu: {spx / c_s:.3f} Mpix/s")
         # the same oracle on every host core (OpenMP over rows), on up to four
         # 1 Mpx frames of the batch

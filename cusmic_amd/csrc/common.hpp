@@ -210,7 +210,7 @@ border_index(int x, int n, int mode)
 	return mode == CUSMIC_REFLECT ? p - 1 - x : p - x;
 }
 
-/* Order-preserving key: -0 below +0 (src/replace.cuh:228-241). */
+/* Order-preserving key: -0 below +0 (src/replace.cuh:37-48). */
 __device__ __forceinline__ unsigned long long
 order_key(double v)
 {
@@ -224,7 +224,7 @@ key_value(unsigned long long k)
 	return __longlong_as_double((long long)((k >> 63) ? k ^ (1ull << 63) : ~k));
 }
 
-/* src/replace.cuh:243-249 */
+/* Overflow-safe midpoint (src/replace.cuh:50-56) */
 __device__ __forceinline__ double
 midpoint(double lo, double hi)
 {

@@ -17,7 +17,11 @@
  *   dilation                     scipy.ndimage.binary_dilation vectors
  *   replacement                  brute-force numpy median + KATs of
  *                                test/test_cleaner.py:135-151
- *   noise model                  closed form checked against numpy in tests
+ *   noise model                  closed-form flag thresholds: raised pixels on
+ *                                flat frames flip at a = t N for scalar and
+ *                                map calibration, the 1e-5 floor, and plateau
+ *                                scenes that separate the 5x5 median from the
+ *                                3x3 and 7x7 ones (tests/test_oracle_noise.py)
  */
 #include "oracle.h"
 #include <math.h>
@@ -172,7 +176,7 @@ oracle_dilate(const uint8_t *in, uint8_t *out, long w, long h)
 }
 
 /* Order-preserving unsigned key of a double: -0 sorts below +0
- * (src/replace.cuh:228-241). */
+ * (src/replace.cuh:37-48). */
 static uint64_t
 order_key(double v)
 {
@@ -196,7 +200,7 @@ cmp_donor(const void *a, const void *b)
 	return (x->index > y->index) - (x->index < y->index);
 }
 
-/* Overflow-safe midpoint (src/replace.cuh:243-249). */
+/* Overflow-safe midpoint (src/replace.cuh:50-56). */
 static double
 midpoint(double lo, double hi)
 {

@@ -6,8 +6,11 @@ Checks, on noise-model frames from the single-frame generator:
     summed in the same order (bit for bit);
   * central finite differences (h = 1e-6 |x|) on random tie-free pixels whose
     mask is unchanged by the perturbation agree to 1e-8 relative;
-  * the background gradient is exactly zero, and finite differences of the
-    loss in the background are rounding noise only.
+  * a scalar background's gradient is exactly zero, and finite differences of
+    the loss in it are rounding noise only;
+  * a background map's gradient (shared by the frames or one per frame)
+    equals the CPU gradient from the oracle's donors bit for bit, and central
+    differences on map pixels: -w at a flagged pixel, +w/2 or +w at a donor.
 """
 
 import numpy as np
@@ -138,3 +141,76 @@ def test_background_gradient_zero(cuda):
     # pure rounding: (D + bg) - bg moves by at most an ulp of D + bg per pixel
     bound = np.sum(np.abs(w) * np.spacing(np.abs(data) + 150)) / (2 * h)
     assert abs(fd) <= bound
+
+
+def cpu_background_gradient(masks, donors, weights, shared):
+    """d loss / d bg from the oracle's donors: a flagged pixel p gives -w_p at
+    p and its donor weights times w_p at the donors; frames are added in order
+    when one map is shared."""
+    grads = cpu_gradient(masks, donors, weights) - np.stack(weights)
+    if not shared:
+        return grads
+    acc = grads[0].copy()
+    for g in grads[1:]:
+        acc = acc + g
+    return acc
+
+
+@pytest.mark.parametrize("shared", [True, False])
+def test_background_map_gradient(cuda, shared):
+    from cusmic_amd import autograd
+
+    data, w = batch(n=2, shape=(80, 96), seed=21)
+    rng = np.random.default_rng(21)
+    bg_np = rng.uniform(50, 250, data.shape[1:] if shared else data.shape)
+    say("background map gradient", "shared" if shared else "per frame")
+    x = torch.tensor(data, device=cuda, requires_grad=True)
+    bg = torch.tensor(bg_np, device=cuda, requires_grad=True)
+    wt = torch.tensor(w, device=cuda)
+    out, mask = autograd.remove_cosmics(x, 3, 5, 3, effective_gain=2.0, readnoise=5.0, background=bg)
+    (wt * out).sum().backward()
+    frames = [dict(data=data[f], background=bg_np if shared else bg_np[f]) for f in range(2)]
+    wo, wm, wd, _ = oracle.clean_frames(frames, gain=2.0, readnoise=5.0, record_donors=True)
+    np.testing.assert_array_equal(mask.cpu().numpy(), np.stack(wm))
+    want_data = cpu_gradient(wm, wd, w)
+    np.testing.assert_array_equal(x.grad.cpu().numpy().view(np.uint64), want_data.view(np.uint64))
+    want = cpu_background_gradient(wm, wd, list(w), shared)
+    got = bg.grad.cpu().numpy()
+    np.testing.assert_array_equal(got.view(np.uint64), want.view(np.uint64))
+    flagged = np.stack(wm)
+    assert flagged.sum() > 20
+    # central differences on map pixels: flagged pixels, donors, others
+    flat_bg = bg_np.ravel()
+    per_frame = data[0].size
+    picks = []
+    for f in range(2):
+        fl = np.flatnonzero(wm[f].ravel())
+        dd = wd[f].reshape(-1, 2)[fl].ravel()
+        base = 0 if shared else f * per_frame
+        picks += [base + i for i in rng.choice(fl, min(20, fl.size), replace=False)]
+        picks += [base + i for i in rng.choice(dd[dd >= 0], 20)]
+    picks += list(rng.integers(0, flat_bg.size, 10))
+    checked = kinds = 0
+    for j in picks:
+        hstep = 1e-6 * abs(flat_bg[j])
+        up, dn = flat_bg.copy(), flat_bg.copy()
+        up[j] += hstep
+        dn[j] -= hstep
+        ou, mu = loss_parts(data, w, cuda, background=up.reshape(bg_np.shape))
+        od, md = loss_parts(data, w, cuda, background=dn.reshape(bg_np.shape))
+        if not (np.array_equal(mu, flagged) and np.array_equal(md, flagged)):
+            continue
+        diff = (ou - od).ravel()
+        changed = np.flatnonzero(diff)
+        ratios = diff[changed] / (up[j] - dn[j])
+        if not np.all(np.isclose(ratios, np.round(ratios * 2) / 2, rtol=0, atol=1e-6)):
+            continue  # a donor order switched within the step: not a fixed point
+        fd = float(np.sum(w.ravel()[changed] * diff[changed])) / (up[j] - dn[j])
+        g = float(got.ravel()[j])
+        # the rounding of (D + bg) - bg at the changed pixels is the only noise
+        noise = 4 * float(np.sum(np.abs(w.ravel()[changed]) *
+                                 np.spacing(np.abs(data.ravel()[changed]) + 300))) / (up[j] - dn[j])
+        assert abs(fd - g) <= 1e-8 * max(1.0, abs(g)) + noise, (j, fd, g)
+        checked += 1
+        kinds += g != 0
+    assert checked >= 60 and kinds >= 30

@@ -1,0 +1,190 @@
+"""The reference's device entry points through the C ABI (GPU).
+
+Restates test/test_batch.cu:80-111: three same-shape frames that share one
+error pointer, cleaned by remove_cosmics_batch_device on a caller stream after
+an event wait on a delayed producer, then frame by frame through
+remove_cosmics_device; the two must agree bit for bit and flag the marked
+rays. Also: the batch's same-shape and data rules (src/api.cu:47-81), the
+NULL stream, and the closed-form noise-model scenes on the GPU.
+"""
+
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from conftest import assert_same_pixels
+from noise_scenes import plateau, plateau_medians, noise, raised_pixels
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*args):
+    print("[case]", *args, flush=True)
+
+
+def ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def image(data, error=None, w=None, h=None):
+    from cusmic_amd import _lib
+
+    return _lib.CusmicImage(data.data_ptr() if data is not None else None,
+                            error.data_ptr() if error is not None else None, None, None, None, None,
+                            w, h)
+
+
+def batch_call(ims, o, clean, mask, stream):
+    from cusmic_amd import _lib
+
+    arr = (_lib.CusmicImage * len(ims))(*ims)
+    msg = ctypes.create_string_buffer(256)
+    st = _lib.lib().remove_cosmics_batch_device(arr, len(ims), ctypes.byref(o), ptr(clean), ptr(mask),
+                                                ctypes.c_void_p(stream), msg, 256)
+    return st, msg.value.decode()
+
+
+def single_call(im, o, clean, mask, stream):
+    from cusmic_amd import _lib
+
+    msg = ctypes.create_string_buffer(256)
+    st = _lib.lib().remove_cosmics_device(ctypes.byref(im), ctypes.byref(o), ptr(clean), ptr(mask),
+                                          ctypes.c_void_p(stream), msg, 256)
+    return st, msg.value.decode()
+
+
+def test_batch_equals_single_frames_on_caller_stream(cuda):
+    from cusmic_amd import _lib
+
+    say("batch device vs single device, shared error pointer, caller stream")
+    w = h = 9
+    n, nf = w * h, 3
+    producer, consumer = torch.cuda.Stream(), torch.cuda.Stream()
+    ready = torch.cuda.Event()
+    data = torch.empty(nf * n, dtype=torch.float64, device=cuda)
+    error = torch.ones(n, dtype=torch.float64, device=cuda)
+    batch = torch.full((nf * n,), -1.0, dtype=torch.float64, device=cuda)
+    flags = torch.full((nf * n,), 7, dtype=torch.uint8, device=cuda)
+    single = torch.empty(n, dtype=torch.float64, device=cuda)
+    one_mask = torch.empty(n, dtype=torch.uint8, device=cuda)
+    torch.cuda.synchronize()
+    with torch.cuda.stream(producer):
+        # the upload is late: the cleaner must wait for the event, not the data
+        torch.cuda._sleep(5_000_000)
+        for f in range(nf):
+            data[f * n:(f + 1) * n] = 10 + f
+            data[f * n + (4 + f % 2) * w + 4 + f // 2] = 1000 + 100 * f
+        ready.record(producer)
+    consumer.wait_event(ready)
+    ims = [image(data[f * n:], error, w, h) for f in range(nf)]
+    o = _lib.default_options()
+    st, msg = batch_call(ims, o, batch, flags, consumer.cuda_stream)
+    assert st == _lib.OK, msg
+    for f in range(nf):
+        st, msg = single_call(ims[f], o, single, one_mask, consumer.cuda_stream)
+        assert st == _lib.OK, msg
+        got, got_m = batch[f * n:(f + 1) * n].cpu().numpy(), flags[f * n:(f + 1) * n].cpu().numpy()
+        assert_same_pixels(got, single.cpu().numpy())
+        np.testing.assert_array_equal(got_m, one_mask.cpu().numpy())
+        ray = (4 + f % 2) * w + 4 + f // 2
+        assert one_mask[ray] == 1 and int(one_mask.sum()) == 1
+        np.testing.assert_array_equal(got, 10 + f)
+    # the oracle agrees on the same frames
+    host = data.cpu().numpy().reshape(nf, h, w)
+    want, wmask, _, _ = oracle.clean_frames([dict(data=host[f], error=np.ones((h, w)))
+                                             for f in range(nf)])
+    assert_same_pixels(batch.cpu().numpy(), np.concatenate([x.ravel() for x in want]))
+    np.testing.assert_array_equal(flags.cpu().numpy().astype(bool),
+                                  np.concatenate([m.ravel() for m in wmask]))
+
+
+def test_batch_rules(cuda):
+    """src/api.cu:47-81: frames of a batch need data and one shape."""
+    from cusmic_amd import _lib
+
+    say("batch shape rules")
+    d = torch.full((2, 9, 9), 10.0, dtype=torch.float64, device=cuda)
+    err = torch.ones((9, 9), dtype=torch.float64, device=cuda)
+    clean = torch.full((2 * 81,), 3.0, dtype=torch.float64, device=cuda)
+    mask = torch.zeros(2 * 81, dtype=torch.uint8, device=cuda)
+    o = _lib.default_options()
+    stream = torch.cuda.current_stream().cuda_stream
+    st, msg = batch_call([image(d[0], err, 9, 9), image(d[1], err, 9, 8)], o, clean, mask, stream)
+    assert st == _lib.INVALID and msg == "frames must have data and the same shape"
+    st, msg = batch_call([image(d[0], err, 9, 9), image(None, err, 9, 9)], o, clean, mask, stream)
+    assert st == _lib.INVALID and msg == "frames must have data and the same shape"
+    st, msg = batch_call([], o, clean, mask, stream)
+    assert st == _lib.INVALID and msg == "invalid images, batch size or output buffers"
+    o.border = 5
+    st, msg = batch_call([image(d[0], err, 9, 9)], o, clean, mask, stream)
+    assert st == _lib.INVALID and msg == "invalid detection settings"
+    # rejected before any work: the device outputs are untouched
+    torch.cuda.synchronize()
+    assert bool((clean == 3.0).all()) and not bool(mask.any())
+
+
+def test_null_stream(cuda):
+    """stream = NULL: the default stream, and still complete on return."""
+    from cusmic_amd import _lib
+
+    say("NULL stream")
+    rng = np.random.default_rng(4)
+    host = rng.normal(100, 3, size=(40, 37))
+    host[10, 10] += 900
+    host[25, 30] += 1500
+    d = torch.tensor(host, device=cuda)
+    err = torch.full_like(d, 3.0)
+    clean = torch.empty(host.size, dtype=torch.float64, device=cuda)
+    mask = torch.empty(host.size, dtype=torch.uint8, device=cuda)
+    torch.cuda.synchronize()
+    o = _lib.default_options()
+    st, msg = single_call(image(d, err, 37, 40), o, clean, mask, 0)
+    assert st == _lib.OK, msg
+    want, wmask, _, _ = oracle.clean_frames([dict(data=host, error=np.full_like(host, 3.0))])
+    assert_same_pixels(clean.cpu().numpy().reshape(40, 37), want[0])
+    np.testing.assert_array_equal(mask.cpu().numpy().reshape(40, 37).astype(bool), wmask[0])
+    assert wmask[0].sum() >= 2
+
+
+def test_noise_model_closed_form(cuda):
+    """The closed-form noise-model scenes: the GPU flags what the published
+    formula predicts (and what the oracle flags)."""
+    from cusmic_amd import remove_cosmics_ragged
+
+    T = 5.0
+    rng = np.random.default_rng(5)
+    shape = (49, 53)
+    gmap = rng.uniform(1.0, 4.0, shape)
+    rnmap = rng.uniform(2.0, 10.0, shape)
+    cases = []
+    for g, rn, c in ((2.0, 5.0, 1000.0), (1.0, 0.0, 0.0), (4.0, 10.0, -30.0), (0.5, 3.0, 1e-6)):
+        data, want = raised_pixels((40, 44), c, g, rn, T, seed=3)
+        cases.append((dict(data=data), g, rn, want))
+    data, want = raised_pixels(shape, 800.0, gmap, rnmap, T, seed=7)
+    cases.append((dict(data=data, gain=gmap, readnoise=rnmap), float("nan"), 0.0, want))
+    for kind in "XY":
+        d, cy, cx = plateau(kind)
+        n5 = noise(plateau_medians(d, cy, cx)[5], 1e-4, 0.0)
+        for fac in (0.95, 1.05):
+            dd = d.copy()
+            dd[cy, cx] += T * n5 * fac
+            want = np.zeros(d.shape, bool)
+            want[cy, cx] = fac > 1
+            cases.append((dict(data=dd), 1e-4, 0.0, want))
+    for k, (frame, g, rn, want) in enumerate(cases):
+        say("closed-form noise scene", k)
+        contrast = 0.25 if k >= 5 else 3.0
+        tn = T if k >= 5 else 3.0
+        tf = {key: torch.tensor(v, device=cuda) for key, v in frame.items()}
+        outs, masks, _, _ = remove_cosmics_ragged([tf], contrast=contrast, cr_threshold=T,
+                                                  neighbor_threshold=tn, maxiter=1,
+                                                  frame_gain=[g], frame_readnoise=[rn])
+        np.testing.assert_array_equal(masks[0].cpu().numpy(), want)
+        wo, wm, _, _ = oracle.clean_frames([frame], contrast=contrast, cr_threshold=T,
+                                           neighbor_threshold=tn, maxiter=1, frame_gain=[g],
+                                           frame_readnoise=[rn])
+        np.testing.assert_array_equal(wm[0], want)
+        assert_same_pixels(outs[0].cpu().numpy(), wo[0])
