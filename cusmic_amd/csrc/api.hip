@@ -4,8 +4,9 @@
  * Host side of the cleaner: argument checks with the reference's messages and
  * status codes (src/api.cu:47-114), the host/device/batch entry points
  * (src/api.cu:155-192) and the ragged-batch/adjoint extensions. One call
- * enqueues the whole loop on the caller's stream and synchronises once at the
- * end to read the device counters (errors, passes).
+ * enqueues the loop on the caller's stream in chunks of passes, reads the
+ * device counters between chunks to stop once every frame has converged, and
+ * synchronises at the end to read the errors and pass counts.
  */
 #include <hip/hip_runtime.h>
 #include <algorithm>
@@ -247,15 +248,17 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	int maxiter = o.maxiter;
 
 	/* one scratch block: frame table | counters (zeroed) | flags | list */
+	/* no part of the scratch grows with maxiter: the per-pass counters are a
+	 * ring of PASS_RING slots */
 	size_t n_counts = nf * (sizeof(Counts) / 4);
 	size_t n_shard = nf * size_t(SHARDS * SHARD_FIELDS);
-	size_t n_todo = size_t(std::max(maxiter, 1)) * nf;
-	size_t n_queue = size_t(std::max(maxiter, 1));
-	size_t n_nlist = 4 + 10 * size_t(std::max(maxiter, 1));
+	size_t n_todo = size_t(PASS_RING) * nf;
+	size_t n_queue = 1;
+	size_t n_nlist = NL_GLOBALS + size_t(PASS_RING) * PASS_FIELDS;
 	size_t words = n_counts + n_shard + n_todo + n_queue + n_nlist + 2 + 5 * size_t(nblocks) + size_t(ntiles);
-	/* the per-pass pixel counts (64-bit) close the zeroed words */
+	/* the processed-pixel count (64-bit) closes the zeroed words */
 	const size_t work_word = align_up(words, 2);
-	words = work_word + 2 * n_queue;
+	words = work_word + 2;
 	size_t off_frames = 0;
 	size_t off_words = align_up(nf * sizeof(Frame), 256);
 	size_t off_dlist = align_up(off_words + words * 4, 256);
@@ -380,7 +383,7 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	int blocks_per_cu = 0;
 	hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, blocks, BLOCK, 0));
 	const unsigned blocks_grid = unsigned(cus * std::max(blocks_per_cu, 1));
-	for (int it = 0; it < maxiter; ++it) {
+	auto enqueue_pass = [&](int it) {
 		/* pass 0: the blocks of tiles with a hole, after the fill; later
 		 * passes: the blocks near a pixel the last replacement changed */
 		if (it > 0)
@@ -400,6 +403,43 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 		hipLaunchKernelGGL(replace_pass, dim3(replace_grid), dim3(BLOCK), 0, stream, p, it, 0);
 		hipLaunchKernelGGL(replace_widened, dim3(cus), dim3(BLOCK), 0, stream, p, it, 0);
 		tm.mark(CUSMIC_PROFILE_REPLACE_MS);
+	};
+	/* The loop in chunks of CHUNK_PASSES passes. Between chunks the host
+	 * reads the pass counters (one synchronisation per chunk) and stops once
+	 * no frame flagged a new pixel in the chunk's last pass, or a replacement
+	 * found no donor. maxiter <= CHUNK_PASSES (the default 4) is one chunk
+	 * and no synchronisation inside the loop. */
+	std::vector<unsigned int> nl(n_nlist);
+	double list_sums[4] = {0, 0, 0, 0}; /* seeds, candidates, grown, widened */
+	auto add_lists = [&](int a, int b) {
+		for (int k = a; k < b; ++k) {
+			list_sums[0] += nl[nl_pass(k, PF_SEED)];
+			list_sums[1] += nl[nl_pass(k, PF_CAND)];
+			list_sums[2] += nl[nl_pass(k, PF_GROW)];
+			list_sums[3] += nl[nl_pass(k, PF_WIDE)];
+		}
+	};
+	int passes_run = 0, listed = 0, status_word = 0;
+	while (passes_run < maxiter) {
+		const int a = passes_run, b = std::min(maxiter, a + CHUNK_PASSES);
+		if (a > 0) {
+			/* the interval since the last event spans the host's read */
+			tm.mark(-1);
+			hipLaunchKernelGGL(clear_pass_slots, dim3(1), dim3(256), 0, stream, p, a + 1, b);
+		}
+		for (int it = a; it < b; ++it)
+			enqueue_pass(it);
+		passes_run = b;
+		if (b == maxiter)
+			break;
+		hip_check(hipGetLastError());
+		hip_check(hipMemcpyAsync(nl.data(), p.nlist, n_nlist * 4, hipMemcpyDeviceToHost, stream));
+		hip_check(hipMemcpyAsync(&status_word, p.status, sizeof(int), hipMemcpyDeviceToHost, stream));
+		hip_check(hipStreamSynchronize(stream));
+		add_lists(listed, b);
+		listed = b;
+		if (!nl[nl_pass(b - 1, PF_ACTIVE)] || (status_word & ERR_NO_DONORS))
+			break;
 	}
 	{
 		int bg_maps = 0;
@@ -412,58 +452,44 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	tm.mark(CUSMIC_PROFILE_EPILOGUE_MS);
 	hip_check(hipGetLastError());
 
-	std::vector<int> host(n_todo + 1);
-	std::vector<unsigned long long> work(n_queue);
-	std::vector<unsigned int> nl(n_nlist);
-	hip_check(hipMemcpyAsync(host.data(), p.todo, n_todo * sizeof(int), hipMemcpyDeviceToHost,
+	std::vector<Counts> counts(nf);
+	unsigned long long work = 0;
+	hip_check(hipMemcpyAsync(counts.data(), p.counts, nf * sizeof(Counts), hipMemcpyDeviceToHost,
 		stream));
-	hip_check(hipMemcpyAsync(host.data() + n_todo, p.status, sizeof(int), hipMemcpyDeviceToHost,
-		stream));
+	hip_check(hipMemcpyAsync(&status_word, p.status, sizeof(int), hipMemcpyDeviceToHost, stream));
 	if (profile) {
-		hip_check(hipMemcpyAsync(work.data(), p.work, n_queue * 8, hipMemcpyDeviceToHost, stream));
+		hip_check(hipMemcpyAsync(&work, p.work, 8, hipMemcpyDeviceToHost, stream));
 		hip_check(hipMemcpyAsync(nl.data(), p.nlist, n_nlist * 4, hipMemcpyDeviceToHost, stream));
 	}
 	hip_check(hipStreamSynchronize(stream));
+	/* frame f flagged new pixels in fresh_passes passes, each followed by a
+	 * replacement; it ran one detection pass more, up to maxiter */
+	auto detect_passes = [&](size_t f) { return std::min(counts[f].fresh_passes + 1, maxiter); };
 	if (profile) {
+		add_lists(listed, passes_run);
 		for (int k = 0; k < CUSMIC_PROFILE_SIZE; ++k)
 			profile[k] = 0;
 		tm.collect(profile);
 		profile[CUSMIC_PROFILE_DETECT_MS] +=
 			profile[CUSMIC_PROFILE_SIGNIFICANCE_MS] + profile[CUSMIC_PROFILE_FIRST_PASS_MS];
-		profile[CUSMIC_PROFILE_DETECT_LAUNCHES] = maxiter;
-		/* frame f ran detection passes until one found nothing new (or
-		 * maxiter), and a replacement pass after each that found pixels */
+		profile[CUSMIC_PROFILE_DETECT_LAUNCHES] = passes_run;
 		for (size_t f = 0; f < nf; ++f) {
 			double px = double(frames[f].w) * frames[f].h;
-			for (int it = 0; it < maxiter; ++it) {
-				profile[CUSMIC_PROFILE_DETECT_PASS_PIXELS] += px;
-				if (!host[size_t(it) * nf + f])
-					break;
-				profile[CUSMIC_PROFILE_REPLACE_PASS_PIXELS] += px;
-			}
+			profile[CUSMIC_PROFILE_DETECT_PASS_PIXELS] += px * detect_passes(f);
+			profile[CUSMIC_PROFILE_REPLACE_PASS_PIXELS] += px * counts[f].fresh_passes;
 		}
-		for (auto v : work)
-			profile[CUSMIC_PROFILE_DETECT_PIXELS] += double(v);
-		profile[CUSMIC_PROFILE_FLAGGED] = nl[0];
-		for (int it = 0; it < maxiter; ++it) {
-			profile[CUSMIC_PROFILE_SEEDS] += nl[2 + maxiter + it];
-			profile[CUSMIC_PROFILE_CANDIDATES] += nl[2 + 2 * maxiter + it];
-			profile[CUSMIC_PROFILE_GROWN] += nl[2 + 3 * maxiter + it];
-			profile[CUSMIC_PROFILE_WIDENED] += nl[2 + 4 * maxiter + it];
-		}
+		profile[CUSMIC_PROFILE_DETECT_PIXELS] = double(work);
+		profile[CUSMIC_PROFILE_FLAGGED] = nl[NL_LIST];
+		profile[CUSMIC_PROFILE_SEEDS] = list_sums[0];
+		profile[CUSMIC_PROFILE_CANDIDATES] = list_sums[1];
+		profile[CUSMIC_PROFILE_GROWN] = list_sums[2];
+		profile[CUSMIC_PROFILE_WIDENED] = list_sums[3];
 	}
-	if (host[n_todo] & ERR_NO_DONORS)
+	if (status_word & ERR_NO_DONORS)
 		throw std::invalid_argument("no finite replacement donors");
 	if (passes)
-		for (size_t f = 0; f < nf; ++f) {
-			int k = 0;
-			for (int it = 0; it < maxiter; ++it) {
-				++k;
-				if (!host[size_t(it) * nf + f])
-					break;
-			}
-			passes[f] = k;
-		}
+		for (size_t f = 0; f < nf; ++f)
+			passes[f] = detect_passes(f);
 }
 
 struct host_result {
@@ -625,20 +651,43 @@ cusmic_adjoint_device(const cusmic_image *ims, size_t nf, const uint8_t *mask,
 		cusmic_default_options(&o);
 		int ntiles = 0;
 		std::vector<Frame> frames = frame_table(ims, nf, o, nullptr, nullptr, &ntiles);
-		size_t off_reach = align_up(nf * sizeof(Frame), 256);
-		scratch mem(off_reach + nf * sizeof(int), s);
+		/* frame table | reach per frame, target count | the first
+		 * ADJ_LIST targets of widened windows */
+		constexpr unsigned ADJ_LIST = 8192;
+		const size_t off_words = align_up(nf * sizeof(Frame), 256);
+		const size_t n_words = nf + 1;
+		const size_t off_list = align_up(off_words + n_words * 4, 256);
+		scratch mem(off_list + ADJ_LIST * 8, s);
 		char *base = static_cast<char *>(mem.ptr);
 		hip_check(hipMemcpyAsync(base, frames.data(), nf * sizeof(Frame), hipMemcpyHostToDevice, s));
-		int *reach = reinterpret_cast<int *>(base + off_reach);
-		hipLaunchKernelGGL(zero_words, dim3(1), dim3(256), 0, s,
-			reinterpret_cast<unsigned int *>(reach), nf);
+		int *reach = reinterpret_cast<int *>(base + off_words);
+		unsigned int *count = reinterpret_cast<unsigned int *>(reach + nf);
+		auto *list = reinterpret_cast<unsigned long long *>(base + off_list);
+		hipLaunchKernelGGL(zero_words, dim3(unsigned((n_words + 255) / 256)), dim3(256), 0, s,
+			reinterpret_cast<unsigned int *>(reach), n_words);
 		const Frame *dframes = reinterpret_cast<const Frame *>(base);
-		hipLaunchKernelGGL(donor_reach, dim3(ntiles), dim3(BLOCK), 0, s, dframes, int(nf), mask,
-			donors, reach);
-		hipLaunchKernelGGL(adjoint_gather, dim3(ntiles), dim3(BLOCK), 0, s, dframes, int(nf), mask,
-			donors, reach, grad_clean, grad_data);
+		hipLaunchKernelGGL(adjoint_gather<true>, dim3(ntiles), dim3(BLOCK), 0, s, dframes, int(nf),
+			mask, donors, grad_clean, grad_data, reach, count, list, ADJ_LIST);
+		const unsigned wide_grid = unsigned(device_cus()) * 2;
+		hipLaunchKernelGGL(adjoint_wide, dim3(wide_grid), dim3(BLOCK), 0, s, dframes, mask, donors,
+			grad_clean, grad_data, reach, count, list, ADJ_LIST);
+		unsigned n = 0;
+		hip_check(hipMemcpyAsync(&n, count, sizeof(n), hipMemcpyDeviceToHost, s));
 		hip_check(hipGetLastError());
 		hip_check(hipStreamSynchronize(s));
+		if (n > ADJ_LIST) {
+			/* more targets than the list held: list them all and redo */
+			scratch all(size_t(n) * 8 + 256, s);
+			auto *big = static_cast<unsigned long long *>(all.ptr);
+			unsigned int *count2 = reinterpret_cast<unsigned int *>(big + n);
+			hipLaunchKernelGGL(zero_words, dim3(1), dim3(256), 0, s, count2, size_t(1));
+			hipLaunchKernelGGL(adjoint_gather<false>, dim3(ntiles), dim3(BLOCK), 0, s, dframes,
+				int(nf), mask, donors, grad_clean, grad_data, reach, count2, big, n);
+			hipLaunchKernelGGL(adjoint_wide, dim3(wide_grid), dim3(BLOCK), 0, s, dframes, mask,
+				donors, grad_clean, grad_data, reach, count2, big, n);
+			hip_check(hipGetLastError());
+			hip_check(hipStreamSynchronize(s));
+		}
 	}, msg, len);
 }
 

@@ -91,9 +91,43 @@ constexpr int CHUNK = 8192;
 constexpr int SHARDS = 32;
 enum { SHARD_HOLES, SHARD_ALLOWED, SHARD_DONORS, SHARD_FRESH, SHARD_FIELDS };
 
-/* Per-frame counters of one call, zeroed at its start. */
+/* Per-frame counters of one call, zeroed at its start. fresh_passes counts
+ * the detection passes that flagged a new pixel: the frame ran fresh_passes
+ * replacement passes and min(fresh_passes + 1, maxiter) detection passes. */
 struct Counts {
-	int holes, allowed, donors, error;
+	int holes, allowed, donors, fresh_passes;
+};
+
+/* The host enqueues the detection loop in chunks of CHUNK_PASSES passes and
+ * stops after a chunk in which no frame flagged a new pixel (the reference
+ * loop's early exit, src/cleaner.cuh:175-181). The per-pass counters live in
+ * a ring of PASS_RING slots: a pass writes its successor's changed-block
+ * count and reads its predecessor's list start, so a slot is cleared before
+ * the chunk that reuses it and stays intact for the pass after it. */
+constexpr int CHUNK_PASSES = 4;
+constexpr int PASS_RING = 2 * CHUNK_PASSES;
+
+/* Params::nlist: global counters, then PASS_FIELDS counters per ring slot. */
+enum {
+	NL_LIST,        /* length of the flagged-pixel list (after the hole list) */
+	NL_HOLES,       /* length of the hole list */
+	NL_WIDE_ALL,    /* length of wide_all */
+	NL_FILL_WIDE,   /* widened replacements of the hole fill */
+	NL_GLOBALS
+};
+enum {
+	PF_DIRTY,       /* dirty blocks of the pass */
+	PF_SEED,        /* seeds */
+	PF_CAND,        /* candidates */
+	PF_GROW,        /* first-growth pixels */
+	PF_WIDE,        /* widened replacements */
+	PF_CHANGED,     /* blocks changed by the previous pass's replacement */
+	PF_START,       /* flagged-list length before the pass's new pixels */
+	PF_REDO,        /* redo list */
+	PF_WIDE_BASE,   /* wide_all length before the pass */
+	PF_EDGE,        /* seeds near a frame edge */
+	PF_ACTIVE,      /* nonzero if some frame flagged a new pixel */
+	PASS_FIELDS
 };
 
 struct Params {
@@ -115,16 +149,10 @@ struct Params {
 	unsigned long long *list;        /* hole list, then the flagged-pixel list */
 	Counts *counts;                  /* per frame */
 	int *shard;                      /* [nf][SHARDS][SHARD_FIELDS] pending counts */
-	int *todo;                       /* [maxiter][nf] new pixels per pass */
-	unsigned int *queue;             /* [maxiter] tile queue heads */
-	unsigned int *nlist;             /* [0] list length, [1] hole count,
-					  * [2 + iter] dirty blocks, [2 + M + iter] seeds,
-					  * [2 + 2M + iter] candidates, [2 + 3M + iter] grown,
-					  * [2 + 4M + iter] widened replacements (iter = M: fill),
-					  * [3 + 5M + iter] changed blocks, [3 + 6M + iter]
-					  * list length before the pass's flags,
-					  * [3 + 7M + iter] redo list, [3 + 8M + iter] wide_all
-					  * length before the pass, [3 + 9M] wide_all length */
+	int *todo;                       /* [PASS_RING][nf] new pixels of the pass
+					  * in each ring slot */
+	unsigned int *queue;             /* run queue head of the first pass */
+	unsigned int *nlist;             /* NL_* globals, then [PASS_RING][PASS_FIELDS] */
 	unsigned int *changed_tag;       /* [nblocks] stamp of the last pass whose
 					  * preceding replacement changed a pixel of
 					  * the block */
@@ -141,7 +169,7 @@ struct Params {
 					  * block pass processes */
 	int *status;                     /* [0] error bits, [1] nonzero if a clean
 					  * sample is infinite (D + bg overflowed) */
-	unsigned long long *work;        /* [maxiter] in-frame pixels of processed tiles */
+	unsigned long long *work;        /* in-frame pixels that detection processed */
 	int *chunk_frame;                /* [nchunks] frame of each chunk */
 	int nchunks;
 	int *run_frame;                  /* [nruns] frame of each first-pass run */
@@ -164,25 +192,54 @@ runs_y(int tiles_y)
 
 constexpr int ERR_NO_DONORS = 1;
 
-/* Stamp of pass iter (0 is never a stamp): S' stamps, block tags. */
+/* S' stamp of pass iter, as kept in the u16 stamp plane and in block_pass (0
+ * is never a stamp). A block pass writes a block's stamp and the stamps of
+ * all its pixels together, so the values may repeat every 65535 passes. */
 __device__ __forceinline__ unsigned
 pass_stamp(int iter)
 {
 	return unsigned(iter % 65535) + 1;
 }
 
+/* Tag of pass iter in the per-block list tags (changed, dirty, reseed): never
+ * 0 and never repeated within a call (iter < maxiter <= INT_MAX). */
+__device__ __forceinline__ unsigned
+tag_stamp(int iter)
+{
+	return unsigned(iter) + 1;
+}
+
+__device__ __host__ __forceinline__ int
+pass_slot(int iter)
+{
+	return iter % PASS_RING;
+}
+
 /* Offsets of the per-pass list counters in Params::nlist. */
-__device__ __forceinline__ int nl_dirty(const Params &p, int it) { return 2 + it; }
-__device__ __forceinline__ int nl_seed(const Params &p, int it) { return 2 + p.maxiter + it; }
-__device__ __forceinline__ int nl_cand(const Params &p, int it) { return 2 + 2 * p.maxiter + it; }
-__device__ __forceinline__ int nl_grow(const Params &p, int it) { return 2 + 3 * p.maxiter + it; }
-__device__ __forceinline__ int nl_wide(const Params &p, int it) { return 2 + 4 * p.maxiter + it; }
-__device__ __forceinline__ int nl_changed(const Params &p, int it) { return 3 + 5 * p.maxiter + it; }
-__device__ __forceinline__ int nl_start(const Params &p, int it) { return 3 + 6 * p.maxiter + it; }
-__device__ __forceinline__ int nl_redo(const Params &p, int it) { return 3 + 7 * p.maxiter + it; }
-__device__ __forceinline__ int nl_wide_base(const Params &p, int it) { return 3 + 8 * p.maxiter + it; }
-__device__ __forceinline__ int nl_wide_all(const Params &p) { return 3 + 9 * p.maxiter; }
-__device__ __forceinline__ int nl_edge(const Params &p, int it) { return 4 + 9 * p.maxiter + it; }
+__device__ __host__ __forceinline__ int
+nl_pass(int iter, int field)
+{
+	return NL_GLOBALS + pass_slot(iter) * PASS_FIELDS + field;
+}
+__device__ __forceinline__ int nl_dirty(const Params &, int it) { return nl_pass(it, PF_DIRTY); }
+__device__ __forceinline__ int nl_seed(const Params &, int it) { return nl_pass(it, PF_SEED); }
+__device__ __forceinline__ int nl_cand(const Params &, int it) { return nl_pass(it, PF_CAND); }
+__device__ __forceinline__ int nl_grow(const Params &, int it) { return nl_pass(it, PF_GROW); }
+__device__ __forceinline__ int nl_wide(const Params &, int it) { return nl_pass(it, PF_WIDE); }
+__device__ __forceinline__ int nl_changed(const Params &, int it) { return nl_pass(it, PF_CHANGED); }
+__device__ __forceinline__ int nl_start(const Params &, int it) { return nl_pass(it, PF_START); }
+__device__ __forceinline__ int nl_redo(const Params &, int it) { return nl_pass(it, PF_REDO); }
+__device__ __forceinline__ int nl_wide_base(const Params &, int it) { return nl_pass(it, PF_WIDE_BASE); }
+__device__ __forceinline__ int nl_edge(const Params &, int it) { return nl_pass(it, PF_EDGE); }
+__device__ __forceinline__ int nl_active(const Params &, int it) { return nl_pass(it, PF_ACTIVE); }
+__device__ __forceinline__ int nl_wide_all(const Params &) { return NL_WIDE_ALL; }
+
+/* Row of pass iter in Params::todo. */
+__device__ __forceinline__ int
+todo_index(const Params &p, int iter, int f)
+{
+	return pass_slot(iter) * p.nf + f;
+}
 
 /* scipy.ndimage border mapping of coordinate x on an axis of n samples;
  * -1 means a CONSTANT-mode zero (src/filters.cuh:24-44). */

@@ -172,8 +172,15 @@ fold_shards(const Params &p, int iter, int first, unsigned t)
 		p.counts[f].holes += d[SHARD_HOLES];
 		p.counts[f].allowed += d[SHARD_ALLOWED];
 		p.counts[f].donors += d[SHARD_DONORS];
-		if (!first)
-			p.todo[iter * p.nf + f] = d[SHARD_FRESH];
+		if (!first) {
+			const int fresh = d[SHARD_FRESH];
+			p.todo[todo_index(p, iter, f)] = fresh;
+			if (fresh > 0) {
+				p.counts[f].fresh_passes += 1;
+				/* every writer stores the same value */
+				__atomic_store_n(&p.nlist[nl_active(p, iter)], 1u, __ATOMIC_RELAXED);
+			}
+		}
 	}
 }
 
@@ -181,6 +188,17 @@ __global__ void
 fold_counts(Params p, int iter, int first)
 {
 	fold_shards(p, iter, first, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+/* Clear the ring slots of passes [lo, hi] before a chunk reuses them (at most
+ * PASS_RING passes; a fill kernel, not a memset, so that the call can be
+ * captured in a graph). */
+__global__ void
+clear_pass_slots(Params p, int lo, int hi)
+{
+	const int n = (hi - lo + 1) * PASS_FIELDS;
+	for (int t = threadIdx.x; t < n; t += blockDim.x)
+		p.nlist[nl_pass(lo + t / PASS_FIELDS, t % PASS_FIELDS)] = 0;
 }
 
 /* Donor records start as (-1, -1): no donor. */
@@ -403,7 +421,7 @@ rank_key(double v, int r, int cnt_neg0)
 __device__ __forceinline__ bool
 changed_block(const Params &p, bool changed, int f, int i, int next, unsigned long long &entry)
 {
-	const unsigned tag = pass_stamp(next);
+	const unsigned tag = tag_stamp(next);
 	unsigned b = 0;
 	bool fresh = false;
 	if (changed) {
@@ -447,7 +465,7 @@ expand_dirty(Params p, int iter)
 {
 	static_assert(REACH < BLK, "a change reaches the adjacent blocks only");
 	__shared__ BlockList<BLOCK * 9> out;
-	const unsigned n = p.nlist[nl_changed(p, iter)], tag = pass_stamp(iter);
+	const unsigned n = p.nlist[nl_changed(p, iter)], tag = tag_stamp(iter);
 	const bool wrap = p.mode == CUSMIC_WRAP;
 	const int r = wrap ? 2 : 1, side = 2 * r + 1;
 	unsigned int *count = &p.nlist[nl_dirty(p, iter)];
@@ -853,7 +871,7 @@ replace_allowed(const Params &p, int f, int iter, int fill)
 		}
 		return true;
 	}
-	if (!p.todo[iter * p.nf + f])
+	if (!p.todo[todo_index(p, iter, f)])
 		return false;
 	if (c.donors <= 0) {
 		atomicOr(p.status, ERR_NO_DONORS);
@@ -881,7 +899,7 @@ replace_pass(Params p, int iter, int fill)
 	const unsigned start = fill ? 0 : p.nlist[nl_start(p, iter)];
 	const unsigned n_new = fill ? p.nlist[1] : p.nlist[0] - start;
 	const unsigned n = n_new + (fill ? 0 : p.nlist[nl_redo(p, iter)]);
-	unsigned int *wide = &p.nlist[nl_wide(p, fill ? p.maxiter : iter)];
+	unsigned int *wide = &p.nlist[fill ? NL_FILL_WIDE : nl_wide(p, iter)];
 	if (!fill && !blockIdx.x && !threadIdx.x)
 		p.nlist[nl_wide_base(p, iter)] = p.nlist[nl_wide_all(p)];
 	for (unsigned base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
@@ -925,7 +943,7 @@ replace_widened(Params p, int iter, int fill)
 {
 	__shared__ unsigned long long ring_keys[BLOCK / 64][64 * RING_CHUNKS];
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-	const unsigned n_pass = p.nlist[nl_wide(p, fill ? p.maxiter : iter)];
+	const unsigned n_pass = p.nlist[fill ? NL_FILL_WIDE : nl_wide(p, iter)];
 	const unsigned n = n_pass + (fill ? 0 : p.nlist[nl_wide_base(p, iter)]);
 	for (unsigned e = blockIdx.x * (BLOCK / 64) + wave; e < n; e += gridDim.x * (BLOCK / 64)) {
 		const bool in_pass = e < n_pass;
@@ -1483,13 +1501,13 @@ significance_pass(SigShared &sh, const Params &p, int iter)
 	const int mode = p.mode;
 	const bool wrap = mode == CUSMIC_WRAP;
 	const double t_cr = p.threshold, t_min = fmin(p.threshold, p.neighbor);
-	const unsigned dtag = pass_stamp(iter);
+	const unsigned dtag = tag_stamp(iter);
 	const unsigned limit = (unsigned)p.nruns;
 	unsigned int *seeds = &p.nlist[nl_seed(p, iter)];
 	unsigned next = 0;
 	unsigned long long work = 0; /* in-frame pixels processed, thread 0 */
 	if (!threadIdx.x) {
-		next = atomicAdd(&p.queue[iter], 1u);
+		next = atomicAdd(p.queue, 1u);
 		sh.nseed = 0;
 		sh.own[0] = sh.own[1] = sh.own[2] = 0;
 	}
@@ -1500,7 +1518,7 @@ significance_pass(SigShared &sh, const Params &p, int iter)
 			unsigned q = next;
 			sh.item = q < limit ? (int)q : -1;
 			if (q < limit)
-				next = atomicAdd(&p.queue[iter], 1u);
+				next = atomicAdd(p.queue, 1u);
 		}
 		lds_barrier();
 		const int item = __builtin_amdgcn_readfirstlane(sh.item);
@@ -1714,7 +1732,7 @@ significance_pass(SigShared &sh, const Params &p, int iter)
 	/* one atomic per workgroup: a counter that every tile hit would
 	 * serialise the whole pass in L2 */
 	if (!threadIdx.x && work)
-		atomicAdd(&p.work[iter], work);
+		atomicAdd(p.work, work);
 }
 
 /* S at an in-frame pixel (x, y); bad = its noise value is invalid. */
@@ -1807,7 +1825,7 @@ significance_blocks_pass(BlockShared &sh, const Params &p, int iter)
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	const int mode = p.mode;
 	const double t_cr = p.threshold, t_min = fmin(p.threshold, p.neighbor);
-	const unsigned stamp = pass_stamp(iter);
+	const unsigned stamp = pass_stamp(iter), tag = tag_stamp(iter);
 	const unsigned n = p.nlist[nl_dirty(p, iter)];
 	unsigned int *seeds = &p.nlist[nl_seed(p, iter)];
 	double *S = sh.s[wave];
@@ -1834,7 +1852,7 @@ significance_blocks_pass(BlockShared &sh, const Params &p, int iter)
 		if ((unsigned)entry & RESEED) {
 			/* S' is still current here: re-emit the seeds, unless the block
 			 * also recomputes in this pass */
-			if (__atomic_load_n(&p.dirty_tag[b], __ATOMIC_RELAXED) == stamp)
+			if (__atomic_load_n(&p.dirty_tag[b], __ATOMIC_RELAXED) == tag)
 				continue;
 			const int x = X0 + lane % BLK, y = Y0 + lane / BLK;
 			const bool seed = x < w && y < h &&
@@ -1959,7 +1977,7 @@ significance_blocks_pass(BlockShared &sh, const Params &p, int iter)
 			total += sh.nseed[k];
 		sh.base = total ? atomicAdd(seeds, total) : 0;
 		if (sh.work)
-			atomicAdd(&p.work[iter], sh.work);
+			atomicAdd(p.work, sh.work);
 	}
 	__syncthreads();
 	unsigned at = sh.base;
@@ -2283,74 +2301,135 @@ grow_second(Params p, int iter)
 /* ------------------------------------------------------------------------ */
 /* Adjoint with the mask held fixed                                          */
 
-/* Chebyshev reach of each frame's recorded donors (>= 2), for the gather. */
-__global__ __launch_bounds__(BLOCK) void
-donor_reach(const Frame *frames, int nf, const uint8_t *mask, const int32_t *donors, int *reach)
+/* The gradient of a donor q is the identity (unflagged q) plus, for every
+ * flagged pixel p whose recorded median donors include q, p's upstream
+ * gradient times its weight (1 for one middle donor, 1/2 for two), added in
+ * raster order of p. The donors of a 5 x 5 window lie within 2 of p; those of
+ * a widened window of radius r all lie at distance exactly r > 2 (the window
+ * widened because no donor was nearer). So adjoint_gather sums over the 5 x 5
+ * neighbourhood of every pixel, and the few targets of widened windows are
+ * listed and recomputed by adjoint_wide over the frame's widest radius. */
+
+constexpr int ADJ_NEAR = 2; /* donor distance of a 5 x 5 window */
+
+/* g[p] times p's weight on donor q (0 if q is not one of p's donors). */
+__device__ __forceinline__ double
+donor_weight_term(const int32_t *donors, const double *grad_clean, long long pg, int q, bool &hit)
 {
-	int tile = blockIdx.x;
-	int f = find_frame(frames, nf, tile);
-	const Frame fr = frames[f];
-	TilePixels t = tile_of(fr, tile);
-	int x = t.x0 + (threadIdx.x & 63), r = 0;
-	const bool col = (threadIdx.x & 63) < TILE_W;
-	for (int row = threadIdx.x >> 6; row < TILE_H; row += 4) {
-		int y = t.y0 + row;
-		if (!col || x >= t.w || y >= t.h)
-			continue;
-		long long g = fr.off + (long long)y * t.w + x;
-		if (!mask[g])
-			continue;
-		for (int k = 0; k < 2; ++k) {
-			int d = donors[2 * g + k];
-			if (d < 0)
-				continue;
-			int dx = abs(d % t.w - x), dy = abs(d / t.w - y);
-			r = max(r, max(dx, dy));
-		}
-	}
-	for (int o = 32; o; o >>= 1)
-		r = max(r, __shfl_xor(r, o, 64));
-	if (!(threadIdx.x & 63) && r > 2)
-		atomicMax(&reach[f], r);
+	const int lo = donors[2 * pg], hi = donors[2 * pg + 1];
+	hit = lo == q || hi == q;
+	return lo == hi ? grad_clean[pg] : 0.5 * grad_clean[pg];
 }
 
-/* grad_data[q] = (q unflagged) * g[q] + sum over flagged p, in raster order of
- * p, of g[p] times p's weight on donor q. */
+/* One workgroup per tile. GATHER: grad_data over the 5 x 5 neighbourhoods.
+ * Both forms append the donor targets of the tile's widened windows
+ * (frame << 32 | q, possibly repeated) to list while fewer than cap are
+ * listed, count them all in *count, and raise reach[f] to the widest
+ * window's radius. */
+template <bool GATHER>
 __global__ __launch_bounds__(BLOCK) void
 adjoint_gather(const Frame *frames, int nf, const uint8_t *mask, const int32_t *donors,
-	const int *reach, const double *grad_clean, double *grad_data)
+	const double *grad_clean, double *grad_data, int *reach, unsigned int *count,
+	unsigned long long *list, unsigned cap)
 {
-	int tile = blockIdx.x;
-	int f = find_frame(frames, nf, tile);
+	const int tile = blockIdx.x;
+	const int f = find_frame(frames, nf, tile);
 	const Frame fr = frames[f];
-	TilePixels t = tile_of(fr, tile);
-	const int R = max(2, reach[f]);
-	int x = t.x0 + (threadIdx.x & 63);
-	const bool col = (threadIdx.x & 63) < TILE_W; /* lanes past the tile width idle */
+	const TilePixels t = tile_of(fr, tile);
+	const int lane = threadIdx.x & 63;
+	const int x = t.x0 + lane;
+	const bool col = lane < TILE_W; /* lanes past the tile width idle */
+	const long long base = fr.off;
+	int widest = 0;
 	for (int row = threadIdx.x >> 6; row < TILE_H; row += 4) {
-		int y = t.y0 + row;
-		if (!col || x >= t.w || y >= t.h)
+		const int y = t.y0 + row;
+		const bool in = col && x < t.w && y < t.h;
+		const int q = in ? y * t.w + x : 0;
+		int targets[2] = {-1, -1};
+		if (in && mask[base + q]) {
+			const int lo = donors[2 * (base + q)], hi = donors[2 * (base + q) + 1];
+			const int r = lo < 0 ? 0 : max(abs(lo % t.w - x), abs(lo / t.w - y));
+			if (r > ADJ_NEAR) {
+				widest = max(widest, r);
+				targets[0] = lo;
+				targets[1] = hi != lo ? hi : -1;
+			}
+		}
+		for (int k = 0; k < 2; ++k) {
+			const bool emit = targets[k] >= 0;
+			const unsigned long long m = __ballot(emit);
+			if (!m)
+				continue;
+			const int leader = __ffsll((long long)m) - 1;
+			unsigned at = 0;
+			if (lane == leader)
+				at = atomicAdd(count, (unsigned)__popcll(m));
+			at = __shfl(at, leader, 64) + __popcll(m & ((1ull << lane) - 1));
+			if (emit && at < cap)
+				list[at] = list_entry(f, targets[k]);
+		}
+		if (!GATHER || !in)
 			continue;
-		int q = y * t.w + x;
-		const long long base = fr.off;
 		double s = mask[base + q] ? 0.0 : grad_clean[base + q];
-		for (int py = max(0, y - R); py <= min(t.h - 1, y + R); ++py)
-			for (int px = max(0, x - R); px <= min(t.w - 1, x + R); ++px) {
-				long long pg = base + (long long)py * t.w + px;
+		for (int py = max(0, y - ADJ_NEAR); py <= min(t.h - 1, y + ADJ_NEAR); ++py)
+			for (int px = max(0, x - ADJ_NEAR); px <= min(t.w - 1, x + ADJ_NEAR); ++px) {
+				const long long pg = base + (long long)py * t.w + px;
 				if (!mask[pg])
 					continue;
-				int lo = donors[2 * pg], hi = donors[2 * pg + 1];
-				if (lo == hi) {
-					if (lo == q)
-						s += grad_clean[pg];
-				} else {
-					if (lo == q)
-						s += 0.5 * grad_clean[pg];
-					if (hi == q)
-						s += 0.5 * grad_clean[pg];
-				}
+				bool hit;
+				const double term = donor_weight_term(donors, grad_clean, pg, q, hit);
+				if (hit)
+					s += term;
 			}
 		grad_data[base + q] = s;
+	}
+	for (int o = 32; o; o >>= 1)
+		widest = max(widest, __shfl_xor(widest, o, 64));
+	if (!lane && widest)
+		atomicMax(&reach[f], widest);
+}
+
+/* One wave per listed target q (entries [0, min(*count, limit))): grad_data[q]
+ * from all flagged pixels within the frame's widest radius, in raster order.
+ * The window is scanned in chunks of 64 positions (lane order is raster
+ * order); the terms found in a chunk are added one after another, so the sum
+ * is the same as a sequential scan. A target listed twice gets the same
+ * value twice. */
+__global__ __launch_bounds__(BLOCK) void
+adjoint_wide(const Frame *frames, const uint8_t *mask, const int32_t *donors,
+	const double *grad_clean, double *grad_data, const int *reach, const unsigned int *count,
+	const unsigned long long *list, unsigned limit)
+{
+	const int lane = threadIdx.x & 63;
+	const unsigned n = min(*count, limit);
+	for (unsigned e = blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6); e < n;
+	     e += gridDim.x * (BLOCK / 64)) {
+		const unsigned long long entry = uniform_u64(list[e]);
+		const int f = (int)(entry >> 32), q = (int)(entry & 0xffffffffu);
+		const Frame &fr = frames[f];
+		const int w = fr.w, h = fr.h, x = q % w, y = q / w, R = reach[f];
+		const int x0 = max(0, x - R), x1 = min(w - 1, x + R), y0 = max(0, y - R), y1 = min(h - 1, y + R);
+		const int ww = x1 - x0 + 1, npos = ww * (y1 - y0 + 1);
+		const long long base = fr.off;
+		double s = mask[base + q] ? 0.0 : grad_clean[base + q];
+		for (int k0 = 0; k0 < npos; k0 += 64) {
+			const int k = k0 + lane;
+			bool hit = false;
+			double term = 0.0;
+			if (k < npos) {
+				const long long pg = base + (long long)(y0 + k / ww) * w + x0 + k % ww;
+				if (mask[pg])
+					term = donor_weight_term(donors, grad_clean, pg, q, hit);
+			}
+			unsigned long long m = __ballot(hit);
+			while (m) {
+				const int j = __ffsll((long long)m) - 1;
+				s += __shfl(term, j, 64);
+				m &= m - 1;
+			}
+		}
+		if (!lane)
+			grad_data[base + q] = s;
 	}
 }
 

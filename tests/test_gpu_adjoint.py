@@ -214,3 +214,80 @@ def test_background_map_gradient(cuda, shared):
         checked += 1
         kinds += g != 0
     assert checked >= 60 and kinds >= 30
+
+
+def wide_scene(kind, h=1024, w=1024, seed=3):
+    """kind "hole": one hot pixel in a masked 41 x 41 square (its replacement
+    window widens to r = 21); kind "grid": hot pixels every 9 px, each in its
+    own masked 7 x 7 square (r = 4), about 25k widened donor targets;
+    kind "plain": the hot pixel of "hole" without the square."""
+    rng = np.random.default_rng(seed)
+    d = rng.normal(1000, 10, size=(h, w))
+    m = np.zeros((h, w), bool)
+    if kind in ("hole", "plain"):
+        cy, cx = 500, 600
+        if kind == "hole":
+            m[cy - 20:cy + 21, cx - 20:cx + 21] = True
+            m[cy, cx] = False
+        d[cy, cx] += 8000
+    else:
+        for cy in range(6, h - 6, 9):
+            for cx in range(6, w - 6, 9):
+                m[cy - 3:cy + 4, cx - 3:cx + 4] = True
+                m[cy, cx] = False
+                d[cy, cx] += 8000
+    return d, m
+
+
+@pytest.mark.parametrize("kind", ["hole", "grid"])
+def test_widened_windows_gradient(cuda, kind):
+    """Donors of widened windows (r = 21; 25k targets, more than the
+    adjoint's first list holds) get their gradient bit for bit as the CPU
+    gradient from the oracle's donors."""
+    from cusmic_amd import autograd
+
+    d, m = wide_scene(kind)
+    say("widened windows", kind)
+    w = np.random.default_rng(8).normal(size=d.shape)
+    x = torch.tensor(d, device=cuda, requires_grad=True)
+    out, mask = autograd.remove_cosmics(x, 3, 5, 3, effective_gain=2.0, readnoise=5.0,
+                                        mask=torch.tensor(m, device=cuda))
+    (torch.tensor(w, device=cuda) * out).sum().backward()
+    wo, wm, wd, _ = oracle.clean_frames([dict(data=d, mask=m)], gain=2.0, readnoise=5.0,
+                                        record_donors=True, threads=8)
+    np.testing.assert_array_equal(mask.cpu().numpy(), wm[0])
+    np.testing.assert_array_equal(out.detach().cpu().numpy().view(np.uint64), wo[0].view(np.uint64))
+    want = cpu_gradient(wm, wd, w[None])[0]
+    np.testing.assert_array_equal(x.grad.cpu().numpy().view(np.uint64), want.view(np.uint64))
+    dd = wd[0].reshape(-1, 2)
+    fl = np.flatnonzero(wm[0].ravel())
+    r = np.maximum(abs(dd[fl, 0] % d.shape[1] - fl % d.shape[1]), abs(dd[fl, 0] // d.shape[1] - fl // d.shape[1]))
+    assert r.max() == (21 if kind == "hole" else 4)
+    if kind == "grid":
+        assert (r > 2).sum() > 8192
+
+
+def test_widened_window_cost(cuda):
+    """The backward of a frame with one r = 21 window costs about what the
+    same frame without it does."""
+    import time
+
+    from cusmic_amd import _lib, engine
+
+    times = {}
+    for kind in ("plain", "hole"):
+        d, m = wide_scene(kind)
+        fr = engine.Frames()
+        fr.add(torch.tensor(d, device=cuda), mask=torch.tensor(m, device=cuda))
+        _, mask, donors, _ = engine.run(fr, cuda, 3.0, 5.0, 3.0, 4, _lib.MIRROR, gain=2.0, readnoise=5.0,
+                                        record_donors=True)
+        g = torch.randn(d.size, dtype=torch.float64, device=cuda)
+        ts = []
+        for _ in range(15):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            engine.adjoint([d.shape], cuda, mask, donors, g)
+            ts.append(time.perf_counter() - t0)
+        times[kind] = float(np.median(ts))
+    say("adjoint times", times)
+    assert times["hole"] < 1.2 * times["plain"] + 3e-5, times

@@ -424,3 +424,59 @@ def test_validation_python(cuda):
             Image(data, **{name: torch.full_like(data, value)})
     with pytest.raises(ValueError, match="Provide error"):
         Cleaner()(Image(data))
+
+
+def test_large_maxiter_stops_at_convergence(cuda):
+    """maxiter far beyond convergence (src/cleaner.cuh:175-181 breaks out of
+    the loop): a scene that needs 14 passes runs through four chunks of the
+    pass-counter ring and equals the oracle; a tiny frame at maxiter = 70000
+    does too."""
+    from cusmic_amd import scenes
+
+    d, sig = scenes.frame((128, 160), 54, "cpu", n_stars=30, n_tracks=90, max_len=12)
+    d = d.numpy()
+    err = scenes.error_map(sig, 2.0, 5.0).numpy()
+    say("14-pass scene, maxiter 100000")
+    got, want = compare([dict(data=d, error=err)], contrast=1.0, cr_threshold=3.0, neighbor_threshold=2.0,
+                        maxiter=100000, record_donors=True)
+    assert want[3][0] == 14, want[3]
+    say("14-pass scene with a converged frame beside it, maxiter 9")
+    d2 = np.full((40, 40), 100.0)
+    d2[20, 20] = 5000
+    got, want = compare([dict(data=d, error=err), dict(data=d2, error=np.full_like(d2, 2.0))],
+                        contrast=1.0, cr_threshold=3.0, neighbor_threshold=2.0, maxiter=9)
+    assert list(want[3]) == [9, 2], want[3]
+    say("tiny frame, maxiter 70000")
+    small = np.full((9, 11), 10.0)
+    small[4, 5] = 1000
+    small[2, 2] = 700
+    got, want = compare([dict(data=small, error=np.ones_like(small))], maxiter=70000)
+    assert want[1][0].sum() == 2 and want[3][0] == 2
+
+
+def test_large_maxiter_costs_no_more(cuda):
+    """A scene that converges within the first chunk costs about the same at
+    maxiter = 1000 as at 4: passes after convergence are not enqueued."""
+    import time
+
+    from cusmic_amd import _lib, engine
+
+    d, err, _ = scene((512, 512), 7, stars=20, tracks=100)
+    fr = engine.Frames()
+    fr.add(torch.tensor(d, device=cuda), error=torch.tensor(err, device=cuda))
+
+    def timed(maxiter):
+        engine.run(fr, cuda, 3.0, 5.0, 3.0, maxiter, _lib.MIRROR)
+        ts = []
+        for _ in range(7):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            _, _, _, passes = engine.run(fr, cuda, 3.0, 5.0, 3.0, maxiter, _lib.MIRROR)
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)), passes[0]
+
+    t4, p4 = timed(4)
+    t1000, p1000 = timed(1000)
+    say("maxiter 4:", t4, p4, "maxiter 1000:", t1000, p1000)
+    assert p4 == p1000 <= 4
+    assert t1000 < 1.5 * t4 + 2e-4, (t4, t1000)
