@@ -18,7 +18,7 @@ MODES = {"reflect": REFLECT, "constant": CONSTANT, "nearest": NEAREST, "mirror":
 EXPORTS = (
     "cusmic_version", "cusmic_cuda_version", "cusmic_default_options", "remove_cosmics",
     "remove_cosmics_device", "remove_cosmics_batch_device", "remove_cosmics_ragged_device",
-    "cusmic_adjoint_device", "cusmic_selftest_arith",
+    "remove_cosmics_batch_device_noalloc", "cusmic_adjoint_device", "cusmic_selftest_arith",
 )
 
 
@@ -99,12 +99,14 @@ def lib():
     h.remove_cosmics.argtypes = [img, opt, p, p, c, sz]
     h.remove_cosmics_device.argtypes = [img, opt, p, p, p, c, sz]
     h.remove_cosmics_batch_device.argtypes = [img, sz, opt, p, p, p, c, sz]
+    h.remove_cosmics_batch_device_noalloc.argtypes = [img, sz, opt, p, p, p, c, sz]
     h.remove_cosmics_ragged_device.argtypes = [img, sz, opt, ctypes.POINTER(RaggedOptions), p, p,
                                                p, c, sz]
     h.cusmic_adjoint_device.argtypes = [img, sz, p, p, p, p, p, c, sz]
     h.cusmic_selftest_arith.argtypes = [p, p, p, p, sz, p]
     for name in ("remove_cosmics", "remove_cosmics_device", "remove_cosmics_batch_device",
-                 "remove_cosmics_ragged_device", "cusmic_adjoint_device", "cusmic_selftest_arith"):
+                 "remove_cosmics_batch_device_noalloc", "remove_cosmics_ragged_device",
+                 "cusmic_adjoint_device", "cusmic_selftest_arith"):
         getattr(h, name).restype = ctypes.c_int
     _lib = h
     return h

@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <vector>
@@ -28,6 +29,12 @@
 #endif
 
 using namespace cusmic;
+
+/* Scratch of remove_cosmics_batch_device_noalloc: one block per device, held
+ * by every device the library runs on until the process ends. Host code uses
+ * it only through its device address (static_scratch), never by this name. */
+constexpr size_t STATIC_SCRATCH_BYTES = size_t(CUSMIC_NOALLOC_SCRATCH_MIB) << 20;
+static __device__ __align__(256) unsigned char scratch[STATIC_SCRATCH_BYTES];
 
 
 namespace {
@@ -113,23 +120,47 @@ check_values(const double *data, size_t n, const char *name, int minimum)
 			throw std::invalid_argument(name);
 }
 
-/* Stream-ordered device scratch released at scope exit. */
-struct scratch {
+/* Stream-ordered device memory released at scope exit. */
+struct device_block {
 	void *ptr = nullptr;
 	hipStream_t stream;
-	scratch(size_t bytes, hipStream_t s) : stream(s)
+	device_block(size_t bytes, hipStream_t s) : stream(s)
 	{
 		if (bytes)
 			hip_check(hipMallocAsync(&ptr, bytes, s));
 	}
-	~scratch()
+	~device_block()
 	{
 		if (ptr)
 			(void)hipFreeAsync(ptr, stream);
 	}
-	scratch(const scratch &) = delete;
-	scratch &operator=(const scratch &) = delete;
+	device_block(const device_block &) = delete;
+	device_block &operator=(const device_block &) = delete;
 };
+
+/* Device address of the static scratch array on the current device, looked
+ * up once per device (each device holds its own copy of the array). */
+constexpr int STATIC_DEVICES = 64;
+std::mutex static_lock;
+void *static_address[STATIC_DEVICES];
+
+void *
+static_scratch()
+{
+	int dev = 0;
+	hip_check(hipGetDevice(&dev));
+	if (dev < 0 || dev >= STATIC_DEVICES)
+		throw std::runtime_error("device index beyond the static scratch table");
+	std::lock_guard<std::mutex> hold(static_lock);
+	if (!static_address[dev]) {
+		void *ptr = nullptr;
+		hip_check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+		if (!ptr)
+			throw std::runtime_error("no device address for the static scratch");
+		static_address[dev] = ptr;
+	}
+	return static_address[dev];
+}
 
 size_t
 align_up(size_t v, size_t a)
@@ -235,10 +266,14 @@ struct timer {
 	}
 };
 
+/* The scratch block is allocated per call, or is the caller's block
+ * (static_base, static_bytes), in which case nothing is allocated and a batch
+ * that needs more is rejected before any work is enqueued. */
 void
 clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const double *fg,
 	const double *frn, int laplacian, double *clean, uint8_t *mask, int32_t *donors,
-	int32_t *passes, hipStream_t stream, double *profile = nullptr)
+	int32_t *passes, hipStream_t stream, double *profile = nullptr, void *static_base = nullptr,
+	size_t static_bytes = 0)
 {
 	int ntiles = 0, nchunks = 0, nruns = 0, nblocks = 0;
 	std::vector<Frame> frames = frame_table(ims, nf, o, fg, frn, &ntiles, &nchunks, &nruns, &nblocks);
@@ -279,8 +314,15 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 		edge += size_t(fr.w) * fr.h - size_t(std::max(fr.w - 8, 0)) * std::max(fr.h - 8, 0);
 	size_t off_edge = align_up(off_wide + planes * 8, 256);
 	size_t bytes = off_edge + (maxiter ? edge : 0) * 8;
-	scratch mem(bytes, stream);
-	char *base = static_cast<char *>(mem.ptr);
+	if (static_base && bytes > static_bytes) {
+		char text[160];
+		std::snprintf(text, sizeof(text),
+			"batch needs %zu bytes of scratch, more than the %zu of the static buffer", bytes,
+			static_bytes);
+		throw std::invalid_argument(text);
+	}
+	device_block mem(static_base ? 0 : bytes, stream);
+	char *base = static_cast<char *>(static_base ? static_base : mem.ptr);
 	hip_check(hipMemcpyAsync(base + off_frames, frames.data(), nf * sizeof(Frame),
 		hipMemcpyHostToDevice, stream));
 
@@ -521,7 +563,7 @@ clean_host(const cusmic_image &im, const cusmic_options &o, size_t n)
 	const double *rnmap = im.error ? nullptr : im.readnoise;
 	size_t planes = 2 + (im.error != nullptr) + (gmap != nullptr) + (rnmap != nullptr) +
 			(im.background != nullptr);
-	scratch mem(planes * n * sizeof(double) + 2 * n + 512, stream);
+	device_block mem(planes * n * sizeof(double) + 2 * n + 512, stream);
 	char *base = static_cast<char *>(mem.ptr);
 	double *d = reinterpret_cast<double *>(base);
 	double *next = d;
@@ -610,6 +652,18 @@ remove_cosmics_batch_device(const cusmic_image *ims, size_t nf, const cusmic_opt
 }
 
 extern "C" int
+remove_cosmics_batch_device_noalloc(const cusmic_image *ims, size_t nf, const cusmic_options *o,
+	double *clean, uint8_t *mask, void *stream, char *msg, size_t len)
+{
+	return guarded([&] {
+		check_arguments(ims, nf, o, clean, mask, true);
+		clean_images(ims, nf, *o, nullptr, nullptr, CUSMIC_LAPLACIAN_EXACT, clean, mask, nullptr,
+			nullptr, static_cast<hipStream_t>(stream), nullptr, static_scratch(),
+			STATIC_SCRATCH_BYTES);
+	}, msg, len);
+}
+
+extern "C" int
 remove_cosmics_device(const cusmic_image *im, const cusmic_options *o, double *clean,
 	uint8_t *mask, void *stream, char *msg, size_t len)
 {
@@ -657,7 +711,7 @@ cusmic_adjoint_device(const cusmic_image *ims, size_t nf, const uint8_t *mask,
 		const size_t off_words = align_up(nf * sizeof(Frame), 256);
 		const size_t n_words = nf + 1;
 		const size_t off_list = align_up(off_words + n_words * 4, 256);
-		scratch mem(off_list + ADJ_LIST * 8, s);
+		device_block mem(off_list + ADJ_LIST * 8, s);
 		char *base = static_cast<char *>(mem.ptr);
 		hip_check(hipMemcpyAsync(base, frames.data(), nf * sizeof(Frame), hipMemcpyHostToDevice, s));
 		int *reach = reinterpret_cast<int *>(base + off_words);
@@ -677,7 +731,7 @@ cusmic_adjoint_device(const cusmic_image *ims, size_t nf, const uint8_t *mask,
 		hip_check(hipStreamSynchronize(s));
 		if (n > ADJ_LIST) {
 			/* more targets than the list held: list them all and redo */
-			scratch all(size_t(n) * 8 + 256, s);
+			device_block all(size_t(n) * 8 + 256, s);
 			auto *big = static_cast<unsigned long long *>(all.ptr);
 			unsigned int *count2 = reinterpret_cast<unsigned int *>(big + n);
 			hipLaunchKernelGGL(zero_words, dim3(1), dim3(256), 0, s, count2, size_t(1));

@@ -62,6 +62,26 @@ int remove_cosmics_ragged_device(const struct cusmic_image *ims, size_t nf,
 	const struct cusmic_options *options, const struct cusmic_ragged_options *ext,
 	double *clean, uint8_t *mask, void *stream, char *msg, size_t message_size);
 
+/* Size of the scratch buffer that remove_cosmics_batch_device_noalloc uses, in
+ * MiB. A call needs about 44 bytes per pixel plus a small table per frame:
+ * at most about 3 million pixels, e.g. one 1024 x 1024 frame (46 MB) or eleven
+ * 512 x 512 frames. */
+#define CUSMIC_NOALLOC_SCRATCH_MIB 128
+
+/* remove_cosmics_batch_device (src/cusmic.h:80-86) without any device memory
+ * allocation: its scratch is one static buffer of CUSMIC_NOALLOC_SCRATCH_MIB
+ * per device, which every device the library runs on holds until the process
+ * ends. No counterpart in the reference; for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation). Same arguments, validation, results, status codes and
+ * completion semantics as remove_cosmics_batch_device; a batch that needs more
+ * scratch than the buffer holds is rejected with CUSMIC_INVALID before any
+ * work. Every call on one device shares that device's buffer: calls that
+ * overlap on one device (several host threads or streams) race. */
+int remove_cosmics_batch_device_noalloc(const struct cusmic_image *ims, size_t nf,
+	const struct cusmic_options *options, double *clean, uint8_t *mask, void *stream, char *msg,
+	size_t message_size);
+
 /* Adjoint of the cleaning map with the detection mask held fixed:
  * grad_data = grad_clean on unflagged pixels, plus, for every flagged pixel,
  * grad_clean scattered to its recorded donors (weight 1 for an odd donor count,
