@@ -56,6 +56,8 @@ class RaggedOptions(ctypes.Structure):
         ("passes", ctypes.c_void_p),
         ("laplacian", ctypes.c_int),
         ("profile", ctypes.c_void_p),
+        ("new_pixels", ctypes.c_void_p),
+        ("new_pixels_size", ctypes.c_size_t),
     ]
 
 

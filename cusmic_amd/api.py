@@ -1,4 +1,9 @@
-"""The L.A.Cosmic call surface (mirrors mod/cusmic/api.py:20-63)."""
+"""Module-level entry points: the L.A.Cosmic call and a ragged-batch call.
+
+``remove_cosmics`` keeps L.A.Cosmic 1.4.0's signature, as the reference's
+does (mod/cusmic/api.py:20-63, checked by test/test_reference.py:46), so that
+code written for the CPU package runs unchanged on tensors.
+"""
 
 import math
 
@@ -22,35 +27,42 @@ def remove_cosmics(
     maxiter=4,
     border_mode="mirror",
 ):
-    """Return cleaned float64 pixels and a boolean cosmic-ray mask on the GPU.
+    """Detect cosmic rays with L.A.Cosmic and replace them; returns
+    ``(cleaned, crmask)``: new float64 and bool tensors on the data's GPU.
 
-    ``data`` is a nonempty float64 frame (height, width) or stack
-    (frames, height, width). NumPy inputs are uploaded; GPU tensors are
-    borrowed. Frames are cleaned independently and outputs do not alias inputs.
+    Input: one frame of shape (height, width) or a stack (frames, height,
+    width) of float64 pixels; the frames of a stack are independent. Host
+    arrays are copied to the GPU, GPU tensors (views too) are read in place and
+    never written.
 
-    Supply a finite, positive 1-sigma ``error`` map in image units, or both
-    ``effective_gain`` (electrons per image unit, positive) and ``readnoise``
-    (electrons, nonnegative). Error overrides gain and read noise. Maps may
-    match data or one shared frame; gain and read noise also accept scalars.
+    Noise: either ``error``, the 1-sigma uncertainty per pixel in data units
+    (finite, > 0), or the noise model from ``effective_gain`` (electrons per
+    data unit, > 0) and ``readnoise`` (electrons, >= 0); with ``error`` the
+    other two are ignored. Maps have the data's shape or one frame's (shared
+    by the stack); gain and read noise may be scalars.
 
-    ``contrast`` rejects fine structure; ``cr_threshold`` and
-    ``neighbor_threshold`` set detection and growth thresholds in sigma.
-    All three must be finite and nonnegative. ``mask`` excludes seed and
-    donor pixels, but neighboring detections may grow into masked pixels.
-    Nonfinite input pixels are preserved and never flagged.
+    Detection: a pixel is a candidate when its significance exceeds
+    ``cr_threshold`` and its ratio to the fine structure exceeds
+    ``contrast``; neighbours of candidates join at ``cr_threshold`` and then
+    at ``neighbor_threshold`` (all three in units of sigma, finite, >= 0).
+    Up to ``maxiter`` detection and replacement passes run, fewer if a pass
+    flags nothing new; ``maxiter = 0`` only applies the background
+    arithmetic and needs no noise inputs.
 
-    ``background`` is a finite scalar or map in image units, added before
-    cleaning and subtracted afterward. ``maxiter`` is nonnegative; zero
-    disables detection and needs no noise model, but still applies background
-    arithmetic. Border modes are mirror, reflect, nearest, wrap, and constant
-    (zero padding).
+    ``mask`` marks pixels that may neither seed a detection nor serve as a
+    replacement donor (growth may still reach them). ``background`` (a
+    finite scalar or map) is added before detection and subtracted from the
+    result. Nonfinite pixels come back unchanged and are never flagged.
+    ``border_mode`` extends the frame for the filters: "mirror", "reflect",
+    "nearest", "wrap" or "constant" (zeros).
 
-    Work uses the input device's current stream. A replacement with no finite,
-    unmasked donors raises ValueError.
+    The work is ordered on the data device's current stream; inputs written
+    on another stream must be made ready with an event first. ValueError is
+    raised for invalid input and when a replacement finds no finite,
+    unmasked donor.
     """
     image = Image(data, error, mask, background, effective_gain, readnoise)
-    clean = Cleaner(contrast, cr_threshold, neighbor_threshold, maxiter, border_mode)
-    return clean(image)
+    return Cleaner(contrast, cr_threshold, neighbor_threshold, maxiter, border_mode)(image)
 
 
 def remove_cosmics_ragged(frames, contrast=3, cr_threshold=5, neighbor_threshold=3, maxiter=4,

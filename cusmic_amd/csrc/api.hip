@@ -273,8 +273,10 @@ void
 clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const double *fg,
 	const double *frn, int laplacian, double *clean, uint8_t *mask, int32_t *donors,
 	int32_t *passes, hipStream_t stream, double *profile = nullptr, void *static_base = nullptr,
-	size_t static_bytes = 0)
+	size_t static_bytes = 0, int64_t *new_pixels = nullptr, size_t new_pixels_size = 0)
 {
+	for (size_t k = 0; new_pixels && k < new_pixels_size; ++k)
+		new_pixels[k] = 0;
 	int ntiles = 0, nchunks = 0, nruns = 0, nblocks = 0;
 	std::vector<Frame> frames = frame_table(ims, nf, o, fg, frn, &ntiles, &nchunks, &nruns, &nblocks);
 	size_t total = 0;
@@ -452,14 +454,24 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	 * found no donor. maxiter <= CHUNK_PASSES (the default 4) is one chunk
 	 * and no synchronisation inside the loop. */
 	std::vector<unsigned int> nl(n_nlist);
+	std::vector<int> todo_ring(new_pixels ? n_todo : 0);
 	double list_sums[4] = {0, 0, 0, 0}; /* seeds, candidates, grown, widened */
+	/* the lists and new pixels of passes [a, b), read from the ring */
 	auto add_lists = [&](int a, int b) {
 		for (int k = a; k < b; ++k) {
 			list_sums[0] += nl[nl_pass(k, PF_SEED)];
 			list_sums[1] += nl[nl_pass(k, PF_CAND)];
 			list_sums[2] += nl[nl_pass(k, PF_GROW)];
 			list_sums[3] += nl[nl_pass(k, PF_WIDE)];
+			if (new_pixels && size_t(k) < new_pixels_size)
+				for (size_t f = 0; f < nf; ++f)
+					new_pixels[k] += todo_ring[size_t(pass_slot(k)) * nf + f];
 		}
+	};
+	auto read_todo = [&]() {
+		if (new_pixels)
+			hip_check(hipMemcpyAsync(todo_ring.data(), p.todo, n_todo * sizeof(int),
+				hipMemcpyDeviceToHost, stream));
 	};
 	int passes_run = 0, listed = 0, status_word = 0;
 	while (passes_run < maxiter) {
@@ -477,6 +489,7 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 		hip_check(hipGetLastError());
 		hip_check(hipMemcpyAsync(nl.data(), p.nlist, n_nlist * 4, hipMemcpyDeviceToHost, stream));
 		hip_check(hipMemcpyAsync(&status_word, p.status, sizeof(int), hipMemcpyDeviceToHost, stream));
+		read_todo();
 		hip_check(hipStreamSynchronize(stream));
 		add_lists(listed, b);
 		listed = b;
@@ -503,7 +516,10 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 		hip_check(hipMemcpyAsync(&work, p.work, 8, hipMemcpyDeviceToHost, stream));
 		hip_check(hipMemcpyAsync(nl.data(), p.nlist, n_nlist * 4, hipMemcpyDeviceToHost, stream));
 	}
+	read_todo();
 	hip_check(hipStreamSynchronize(stream));
+	if (new_pixels && !profile)
+		add_lists(listed, passes_run);
 	/* frame f flagged new pixels in fresh_passes passes, each followed by a
 	 * replacement; it ran one detection pass more, up to maxiter */
 	auto detect_passes = [&](size_t f) { return std::min(counts[f].fresh_passes + 1, maxiter); };
@@ -684,7 +700,8 @@ remove_cosmics_ragged_device(const cusmic_image *ims, size_t nf, const cusmic_op
 		check_arguments(ims, nf, o, clean, mask, false, fg, frn);
 		clean_images(ims, nf, *o, fg, frn, lap, clean, mask, ext ? ext->donors : nullptr,
 			ext ? ext->passes : nullptr, static_cast<hipStream_t>(stream),
-			ext ? ext->profile : nullptr);
+			ext ? ext->profile : nullptr, nullptr, 0, ext ? ext->new_pixels : nullptr,
+			ext ? ext->new_pixels_size : 0);
 	}, msg, len);
 }
 

@@ -60,10 +60,12 @@ class Frames:
 def run(frames, device, contrast, cr_threshold, neighbor_threshold, maxiter, border,
         gain=math.nan, readnoise=0.0, background=math.nan, frame_gain=None,
         frame_readnoise=None, record_donors=False, laplacian=_lib.LAPLACIAN_EXACT,
-        clean=None, mask=None, profile=None):
+        clean=None, mask=None, profile=None, new_pixels=None):
     """Clean the frames in one call. Returns (clean, mask_u8, donors, passes) as
     flat device tensors (donors None unless recorded) and per-frame passes.
-    A dict passed as ``profile`` receives the library's kernel timings."""
+    A dict passed as ``profile`` receives the library's kernel timings; a list
+    passed as ``new_pixels`` receives the new pixels of each pass (over all
+    frames), one per element."""
     h = _lib.lib()
     nf = len(frames.images)
     total = frames.total
@@ -79,11 +81,14 @@ def run(frames, device, contrast, cr_threshold, neighbor_threshold, maxiter, bor
     fg = (ctypes.c_double * nf)(*frame_gain) if frame_gain is not None else None
     frn = (ctypes.c_double * nf)(*frame_readnoise) if frame_readnoise is not None else None
     prof = (ctypes.c_double * len(_lib.PROFILE_FIELDS))() if profile is not None else None
+    fresh = (ctypes.c_int64 * len(new_pixels))() if new_pixels else None
     ext = _lib.RaggedOptions(
         ctypes.cast(fg, ctypes.c_void_p) if fg is not None else None,
         ctypes.cast(frn, ctypes.c_void_p) if frn is not None else None,
         _ptr(donors), ctypes.cast(passes, ctypes.c_void_p), laplacian,
-        ctypes.cast(prof, ctypes.c_void_p) if prof is not None else None)
+        ctypes.cast(prof, ctypes.c_void_p) if prof is not None else None,
+        ctypes.cast(fresh, ctypes.c_void_p) if fresh is not None else None,
+        len(new_pixels) if fresh is not None else 0)
     msg = ctypes.create_string_buffer(256)
     with torch.cuda.device(device):
         stream = torch.cuda.current_stream(device).cuda_stream
@@ -94,6 +99,8 @@ def run(frames, device, contrast, cr_threshold, neighbor_threshold, maxiter, bor
         _raise(status, msg)
     if profile is not None:
         profile.update(zip(_lib.PROFILE_FIELDS, list(prof)))
+    if fresh is not None:
+        new_pixels[:] = list(fresh)
     return clean, mask, donors, list(passes)
 
 

@@ -30,6 +30,10 @@ struct cusmic_ragged_options {
 	double *profile;    /* host, CUSMIC_PROFILE_SIZE values, or NULL: when set,
 	                       the call times its kernels with hipEvents on the
 	                       caller's stream (see enum cusmic_profile) */
+	int64_t *new_pixels; /* host, new_pixels_size values, or NULL: pixels
+	                       flagged for the first time in each detection pass,
+	                       over all frames (0 for passes that did not run) */
+	size_t new_pixels_size;
 };
 
 /* Slots of cusmic_ragged_options.profile. Times are summed over launches in
@@ -52,8 +56,10 @@ enum cusmic_profile {
 };
 
 /* Ragged batch: 1..65535 contiguous device frames of individual shapes,
- * cleaned in one call on the caller's stream (persistent tile-queue kernels,
- * no host round trip inside the loop). Outputs are the frames concatenated in
+ * cleaned in one call on the caller's stream (persistent tile-queue kernels;
+ * the host enqueues the passes in chunks of four and, between chunks, reads
+ * one word to stop once no frame flags a new pixel, so maxiter <= 4 makes no
+ * round trip inside the loop). Outputs are the frames concatenated in
  * descriptor order. Per-frame scalars in ext replace options->gain/readnoise.
  * Same validation, status codes and completion semantics as
  * remove_cosmics_batch_device (src/cusmic.h:80-85), which is the special
@@ -85,11 +91,14 @@ int remove_cosmics_batch_device_noalloc(const struct cusmic_image *ims, size_t n
 /* Adjoint of the cleaning map with the detection mask held fixed:
  * grad_data = grad_clean on unflagged pixels, plus, for every flagged pixel,
  * grad_clean scattered to its recorded donors (weight 1 for an odd donor count,
- * 1/2 each for the two middle donors of an even count). Written as a
- * deterministic gather: no atomics, fixed summation order. The gradient with
- * respect to the background is exactly zero and is not computed. Frames,
- * mask and donors are laid out as returned by remove_cosmics_ragged_device;
- * only width/height of ims are read. No counterpart in the reference. */
+ * 1/2 each for the two middle donors of an even count), summed in raster order
+ * of the flagged pixels: a deterministic gather, no atomics on the gradient.
+ * The cost is that of a 5 x 5 gather plus, for the donors of windows wider than
+ * 5 x 5, a rescan of the frame's widest window per donor. The gradient with
+ * respect to a background map is grad_data - grad_clean (a scalar
+ * background's is exactly zero); the caller forms it. Frames, mask and donors
+ * are laid out as returned by remove_cosmics_ragged_device; only width/height
+ * of ims are read. No counterpart in the reference. */
 int cusmic_adjoint_device(const struct cusmic_image *ims, size_t nf, const uint8_t *mask,
 	const int32_t *donors, const double *grad_clean, double *grad_data, void *stream, char *msg,
 	size_t message_size);

@@ -480,3 +480,34 @@ def test_large_maxiter_costs_no_more(cuda):
     say("maxiter 4:", t4, p4, "maxiter 1000:", t1000, p1000)
     assert p4 == p1000 <= 4
     assert t1000 < 1.5 * t4 + 2e-4, (t4, t1000)
+
+
+def test_iteration_log(cuda, caplog):
+    """mod/cusmic/cleaner.py:108 logs each pass's new pixels over the stack,
+    up to the pass that finds none."""
+    import logging
+
+    from cusmic_amd import remove_cosmics
+
+    say("iteration log")
+    stack = torch.full((2, 15, 15), 10.0, dtype=torch.float64, device=cuda)
+    stack[0, 5:10, 5:10] = 1000
+    stack[1, 3, 11] = 900
+    with caplog.at_level(logging.INFO, logger="cusmic_amd.cleaner"):
+        clean, mask = remove_cosmics(stack, 3, 5, 3, error=torch.ones_like(stack))
+    lines = [r.getMessage() for r in caplog.records if r.name == "cusmic_amd.cleaner"]
+    # pass k's new pixels from the oracle: the first k passes are a run with
+    # maxiter = k, so its mask grows by exactly those pixels
+    host = np_of(stack)
+    frames = [dict(data=host[f], error=np.ones((15, 15))) for f in range(2)]
+    want, flagged, k = [], 0, 0
+    while True:
+        k += 1
+        _, masks, _, _ = oracle.clean_frames(frames, maxiter=k)
+        now = int(sum(m.sum() for m in masks))
+        want.append(f"Iteration {k}: {now - flagged} new cosmic-ray pixels")
+        if now == flagged:
+            break
+        flagged = now
+    assert lines == want, (lines, want)
+    assert int(mask.sum()) == flagged and len(want) > 2
