@@ -48,6 +48,7 @@ class _Clean(torch.autograd.Function):
         ctx.stack_shape = data.shape
         ctx.bg_shape = None if background is None else background.shape
         ctx.mark_non_differentiable(mask)
+        ctx.set_materialize_grads(False)
         return clean, mask
 
     @staticmethod
@@ -55,6 +56,8 @@ class _Clean(torch.autograd.Function):
         mask, donors = ctx.saved_tensors
         shape = ctx.stack_shape
         grad_data = grad_bg = None
+        if grad_clean is None:  # the cleaned pixels do not reach the loss
+            return None, None, None, None
         map_bg = ctx.needs_input_grad[1] and len(ctx.bg_shape) > 0
         if ctx.needs_input_grad[0] or map_bg:
             h, w = shape[-2:]

@@ -2325,63 +2325,88 @@ donor_weight_term(const int32_t *donors, const double *grad_clean, long long pg,
  * Both forms append the donor targets of the tile's widened windows
  * (frame << 32 | q, possibly repeated) to list while fewer than cap are
  * listed, count them all in *count, and raise reach[f] to the widest
- * window's radius. */
+ * window's radius.
+ *
+ * Lane l is column x0 - 2 + l, so a wave's ballot over a row is the row's
+ * flagged pixels around the tile's 60 columns (lanes 2..61). Each wave walks
+ * its 8 rows with the masks of the five rows around the current one: a pixel
+ * whose 5 x 5 window holds no flagged pixel (nearly all) gets the identity,
+ * the others add the terms of the flagged pixels of the window, whose bits
+ * are taken in raster order. */
 template <bool GATHER>
 __global__ __launch_bounds__(BLOCK) void
 adjoint_gather(const Frame *frames, int nf, const uint8_t *mask, const int32_t *donors,
 	const double *grad_clean, double *grad_data, int *reach, unsigned int *count,
 	unsigned long long *list, unsigned cap)
 {
+	static_assert(TILE_W + 2 * ADJ_NEAR <= 64, "a tile row and its window fit one wave");
 	const int tile = blockIdx.x;
 	const int f = find_frame(frames, nf, tile);
 	const Frame fr = frames[f];
 	const TilePixels t = tile_of(fr, tile);
-	const int lane = threadIdx.x & 63;
-	const int x = t.x0 + lane;
-	const bool col = lane < TILE_W; /* lanes past the tile width idle */
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int x = t.x0 - ADJ_NEAR + lane;
+	const bool col_in = x >= 0 && x < t.w;
+	const bool out_col = lane >= ADJ_NEAR && lane < ADJ_NEAR + TILE_W && x < t.w;
 	const long long base = fr.off;
+	const int ya = t.y0 + wave * (TILE_H / 4);
+	auto row_mask = [&](int yy) {
+		return __ballot(col_in && yy >= 0 && yy < t.h && mask[base + (long long)yy * t.w + x] != 0);
+	};
+	unsigned long long m[5];
+#pragma unroll
+	for (int k = 0; k < 4; ++k)
+		m[k] = row_mask(ya - ADJ_NEAR + k);
 	int widest = 0;
-	for (int row = threadIdx.x >> 6; row < TILE_H; row += 4) {
-		const int y = t.y0 + row;
-		const bool in = col && x < t.w && y < t.h;
-		const int q = in ? y * t.w + x : 0;
+	for (int r = 0; r < TILE_H / 4; ++r) {
+		const int y = ya + r;
+		m[4] = row_mask(y + ADJ_NEAR);
+		const bool out = out_col && y < t.h;
+		const int q = y * t.w + x;
+		const bool flagged = out && ((m[2] >> lane) & 1);
 		int targets[2] = {-1, -1};
-		if (in && mask[base + q]) {
+		if (flagged) {
 			const int lo = donors[2 * (base + q)], hi = donors[2 * (base + q) + 1];
-			const int r = lo < 0 ? 0 : max(abs(lo % t.w - x), abs(lo / t.w - y));
-			if (r > ADJ_NEAR) {
-				widest = max(widest, r);
+			const int d = lo < 0 ? 0 : max(abs(lo % t.w - x), abs(lo / t.w - y));
+			if (d > ADJ_NEAR) {
+				widest = max(widest, d);
 				targets[0] = lo;
 				targets[1] = hi != lo ? hi : -1;
 			}
 		}
 		for (int k = 0; k < 2; ++k) {
 			const bool emit = targets[k] >= 0;
-			const unsigned long long m = __ballot(emit);
-			if (!m)
+			const unsigned long long b = __ballot(emit);
+			if (!b)
 				continue;
-			const int leader = __ffsll((long long)m) - 1;
+			const int leader = __ffsll((long long)b) - 1;
 			unsigned at = 0;
 			if (lane == leader)
-				at = atomicAdd(count, (unsigned)__popcll(m));
-			at = __shfl(at, leader, 64) + __popcll(m & ((1ull << lane) - 1));
+				at = atomicAdd(count, (unsigned)__popcll(b));
+			at = __shfl(at, leader, 64) + __popcll(b & ((1ull << lane) - 1));
 			if (emit && at < cap)
 				list[at] = list_entry(f, targets[k]);
 		}
-		if (!GATHER || !in)
-			continue;
-		double s = mask[base + q] ? 0.0 : grad_clean[base + q];
-		for (int py = max(0, y - ADJ_NEAR); py <= min(t.h - 1, y + ADJ_NEAR); ++py)
-			for (int px = max(0, x - ADJ_NEAR); px <= min(t.w - 1, x + ADJ_NEAR); ++px) {
-				const long long pg = base + (long long)py * t.w + px;
-				if (!mask[pg])
-					continue;
-				bool hit;
-				const double term = donor_weight_term(donors, grad_clean, pg, q, hit);
-				if (hit)
-					s += term;
+		if (GATHER && out) {
+			double s = flagged ? 0.0 : grad_clean[base + q];
+#pragma unroll
+			for (int k = 0; k < 5; ++k) {
+				unsigned bits = unsigned(m[k] >> (lane - ADJ_NEAR)) & 0x1fu;
+				while (bits) {
+					const int c = __builtin_ctz(bits);
+					bits &= bits - 1;
+					const long long pg = base + (long long)(y - ADJ_NEAR + k) * t.w + x - ADJ_NEAR + c;
+					bool hit;
+					const double term = donor_weight_term(donors, grad_clean, pg, q, hit);
+					if (hit)
+						s += term;
+				}
 			}
-		grad_data[base + q] = s;
+			grad_data[base + q] = s;
+		}
+#pragma unroll
+		for (int k = 0; k < 4; ++k)
+			m[k] = m[k + 1];
 	}
 	for (int o = 32; o; o >>= 1)
 		widest = max(widest, __shfl_xor(widest, o, 64));

@@ -37,9 +37,12 @@ def _checked_calibration(name, value, device, shapes):
     """Validate one calibration field; returns it as a float64 tensor.
 
     A map must have the data's shape or one frame's (``shapes``); apart from
-    the error map a field may also be a 0-d scalar.
+    the error map a field may also be a 0-d scalar. Maps live on the device;
+    scalars stay on the host, so that neither their checks nor their use by
+    the cleaner waits for the device.
     """
-    t = _on_device(value, device)
+    t = value if isinstance(value, torch.Tensor) else torch.as_tensor(np.asarray(value))
+    t = t.detach().cpu() if t.dim() == 0 else _on_device(t, device)
     if t.dtype == torch.bool or t.is_complex():
         raise TypeError(f"{name} must be real numeric data")
     scalar_ok = name != "error" and t.dim() == 0
