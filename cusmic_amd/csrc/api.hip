@@ -215,7 +215,7 @@ device_cus()
 std::vector<Frame>
 frame_table(const cusmic_image *ims, size_t nf, const cusmic_options &o, const double *fg,
 	const double *frn, int *ntiles_out, int *nchunks_out = nullptr, int *nruns_out = nullptr,
-	int *nblocks_out = nullptr)
+	int *nblocks_out = nullptr, int run_tiles = RUN_TILES)
 {
 	std::vector<Frame> frames(nf);
 	long long off = 0;
@@ -249,7 +249,7 @@ frame_table(const cusmic_image *ims, size_t nf, const cusmic_options &o, const d
 		blocks += int(nb);
 		tiles += fr.tiles_x * fr.tiles_y;
 		chunks += int((px + CHUNK - 1) / CHUNK);
-		runs += fr.tiles_x * runs_y(fr.tiles_y);
+		runs += fr.tiles_x * runs_y(fr.tiles_y, run_tiles);
 		off += px;
 	}
 	*ntiles_out = tiles;
@@ -311,8 +311,41 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 {
 	for (size_t k = 0; new_pixels && k < new_pixels_size; ++k)
 		new_pixels[k] = 0;
+	/* the first pass's kernel, and how many of its workgroups fit at once */
+	bool model = false; /* some frame uses the noise model instead of an error map */
+	bool bg_any = !std::isnan(o.background); /* some frame has a background */
+	for (size_t f = 0; f < nf; ++f) {
+		model |= ims[f].error == nullptr;
+		bg_any |= ims[f].background != nullptr;
+	}
+	using kernel_t = void (*)(Params, int);
+	kernel_t sig;
+	if (model)
+		sig = laplacian ? (bg_any ? significance_model<false, true> : significance_model<false, false>)
+				: (bg_any ? significance_model<true, true> : significance_model<true, false>);
+	else
+		sig = laplacian ? (bg_any ? significance_maps<false, true> : significance_maps<false, false>)
+				: (bg_any ? significance_maps<true, true> : significance_maps<true, false>);
+	const int cus = device_cus();
+	int sig_per_cu = 0;
+	hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&sig_per_cu, sig, BLOCK, 0));
+	const int sig_resident = cus * std::max(sig_per_cu, 1);
+	/* first-pass runs: the longest that still gives every resident
+	 * workgroup four runs on average */
+	int run_tiles = RUN_TILES;
+	for (; run_tiles > 1; run_tiles /= 2) {
+		long long runs = 0;
+		for (size_t f = 0; f < nf; ++f) {
+			const long long tx = ((long long)ims[f].width + TILE_W - 1) / TILE_W;
+			const long long ty = ((long long)ims[f].height + TILE_H - 1) / TILE_H;
+			runs += tx * ((ty + run_tiles - 1) / run_tiles);
+		}
+		if (runs >= 4LL * sig_resident)
+			break;
+	}
 	int ntiles = 0, nchunks = 0, nruns = 0, nblocks = 0;
-	std::vector<Frame> frames = frame_table(ims, nf, o, fg, frn, &ntiles, &nchunks, &nruns, &nblocks);
+	std::vector<Frame> frames =
+		frame_table(ims, nf, o, fg, frn, &ntiles, &nchunks, &nruns, &nblocks, run_tiles);
 	size_t total = 0;
 	for (const auto &fr : frames)
 		total += size_t(fr.w) * fr.h;
@@ -369,6 +402,7 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	p.maxiter = maxiter;
 	p.mode = o.border;
 	p.laplacian = laplacian;
+	p.run_tiles = run_tiles;
 	p.contrast = o.contrast;
 	p.threshold = o.cr_threshold;
 	p.neighbor = o.neighbor_threshold;
@@ -415,13 +449,6 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 		return;
 	}
 
-	int cus = device_cus();
-	bool model = false; /* some frame uses the noise model instead of an error map */
-	bool bg_any = !std::isnan(o.background); /* some frame has a background */
-	for (size_t f = 0; f < nf; ++f) {
-		model |= ims[f].error == nullptr;
-		bg_any |= ims[f].background != nullptr;
-	}
 	p.reach = model ? REACH : REACH_MAPS;
 	p.reach_s = model ? REACH_S : REACH_S_MAPS;
 	timer tm(stream, profile != nullptr);
@@ -436,17 +463,6 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	tm.mark(CUSMIC_PROFILE_PROLOGUE_MS);
 	/* the first pass, with the prologue's work, in persistent workgroups: as
 	 * many as fit on the device */
-	using kernel_t = void (*)(Params, int);
-	kernel_t sig;
-	if (model)
-		sig = laplacian ? (bg_any ? significance_model<false, true> : significance_model<false, false>)
-				: (bg_any ? significance_model<true, true> : significance_model<true, false>);
-	else
-		sig = laplacian ? (bg_any ? significance_maps<false, true> : significance_maps<false, false>)
-				: (bg_any ? significance_maps<true, true> : significance_maps<true, false>);
-	int sig_per_cu = 0;
-	hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&sig_per_cu, sig, BLOCK, 0));
-	const int sig_resident = cus * std::max(sig_per_cu, 1);
 	hipLaunchKernelGGL(sig, dim3(unsigned(std::min(nruns, sig_resident))), dim3(BLOCK), 0, stream, p, 0);
 	tm.mark(CUSMIC_PROFILE_FIRST_PASS_MS);
 	const unsigned fold_grid = unsigned((nf * SHARDS + 255) / 256);

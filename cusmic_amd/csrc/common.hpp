@@ -22,7 +22,9 @@ constexpr int BLOCK = 256;
 
 /* The first detection pass walks each frame in runs of up to RUN_TILES
  * vertically adjacent tiles, so that the significance rows shared by two
- * tiles of a run are computed once. */
+ * tiles of a run are computed once. A call with few tiles uses shorter runs
+ * (Params::run_tiles, a power of two), so that its workgroups get several
+ * runs each and the last ones do not finish long after the others. */
 constexpr int RUN_TILES = 8;
 
 /* Passes after the first recompute detection on BLK x BLK blocks marked
@@ -133,6 +135,7 @@ enum {
 struct Params {
 	const Frame *frames;
 	int nf, ntiles, maxiter, mode, laplacian;
+	int run_tiles;                   /* tiles per first-pass run (<= RUN_TILES) */
 	int reach, reach_s;              /* REACH_MAPS or REACH, REACH_S_MAPS or REACH_S */
 	double contrast, threshold, neighbor, bg;  /* bg: scalar background or NaN */
 	double *clean;
@@ -182,12 +185,12 @@ struct Params {
 					  * fill */
 };
 
-/* Tiles per run, in run order: frame-major, then row-major over blocks of
- * RUN_TILES tile rows. */
+/* Runs per tile column of a frame; runs are numbered frame-major, then
+ * row-major over blocks of run_tiles tile rows. */
 __device__ __host__ __forceinline__ int
-runs_y(int tiles_y)
+runs_y(int tiles_y, int run_tiles)
 {
-	return (tiles_y + RUN_TILES - 1) / RUN_TILES;
+	return (tiles_y + run_tiles - 1) / run_tiles;
 }
 
 constexpr int ERR_NO_DONORS = 1;

@@ -1028,7 +1028,7 @@ redo_scan(Params p, int iter)
  *   significance  S = L / (2 N) over a region and a 2-pixel halo in LDS;
  *                 S' = S - med5(S) only where S > min(t, t_n) (S' <= S because
  *                 S >= 0), written to a sparse plane; seeds (S' > t) go to a
- *                 list. The first pass is dense: it walks runs of RUN_TILES
+ *                 list. The first pass is dense: it walks runs of run_tiles
  *                 tiles down a tile column, so rows shared by two tiles are
  *                 computed once, and it makes C from the input as it goes (the
  *                 prologue). The block pass processes the BLK x BLK blocks
@@ -1296,30 +1296,83 @@ own_row(const Params &p, const Frame &fr, bool own, long long i, double c, bool 
 	}
 }
 
-/* One row of S under the noise model, whose 5x5 median of C is med; hm: a
- * hole among the median's samples. */
-template <bool EXACT, bool BG>
+/* The samples of C that one step of the noise-model first pass reads, in LDS:
+ * region row k is run row rS0 - HS + k (rS0: the step's first S row), region
+ * column c is frame column X0 - 2 HS + c, both under the median's border
+ * mode (CONSTANT: 0). The first step has STEP_ROWS S rows and reads the most:
+ * its last wave's last four-row strip ends three rows below them. */
+constexpr int CREG_COLS = SW + 2 * HS;                /* 68 */
+constexpr int CREG_ROWS = STEP_ROWS + 2 * HS + 4;     /* 44 */
+static_assert(CREG_ROWS <= 64, "one bit per region row");
+
+struct ModelRegion {
+	double c[CREG_ROWS][CREG_COLS];
+	unsigned long long hole_lo[CREG_ROWS]; /* columns 0..63 that are holes */
+	unsigned long long hole_hi[CREG_ROWS]; /* columns 64.. that are holes */
+};
+
+/* Load region rows [0, nrows) for the frame rows from yc0 (whole workgroup;
+ * wave w loads rows w, w + 4, ..., lane l columns l and 64 + l). */
+template <bool BG>
 __device__ __forceinline__ void
-model_row(SigShared &sh, int &slot, int lane, double t_min, unsigned long long cols, const Params &p,
-	const Frame &fr, int f, int gx, int gy, double med, bool hm, bool own, OwnCounts &acc)
+load_region(ModelRegion &R, const Params &p, const Frame &fr, int X0, int yc0, int nrows)
 {
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	const int mode = p.mode, w = fr.w, h = fr.h;
 	const bool has_bg = BG && (fr.background || !isnan(p.bg));
-	bool hc = false, hn = hm;
-	const double cc = lap_input<BG>(fr, p.bg, has_bg, gx, gy, mode, hc);
-	const double qu = lap_input<BG>(fr, p.bg, has_bg, gx, gy - 1, mode, hn) * 0.25;
-	const double qd = lap_input<BG>(fr, p.bg, has_bg, gx, gy + 1, mode, hn) * 0.25;
-	const double ql = lap_input<BG>(fr, p.bg, has_bg, gx - 1, gy, mode, hn) * 0.25;
-	const double qr = lap_input<BG>(fr, p.bg, has_bg, gx + 1, gy, mode, hn) * 0.25;
-	int mi = map_index(gx, gy, w, h, mode);
-	double gg = fr.gain ? fr.gain[mi] : fr.g;
-	double rn = fr.readnoise ? fr.readnoise[mi] : fr.rn;
-	double nz = noise_model(med, gg, rn);
-	bool in = gx >= 0 && gx < w && gy >= 0 && gy < h;
+	const int bx0 = border_index(X0 - 2 * HS + lane, w, mode);
+	const bool second = lane < CREG_COLS - 64;
+	const int bx1 = second ? border_index(X0 - 2 * HS + 64 + lane, w, mode) : -1;
+	for (int k = wave; k < nrows; k += BLOCK / 64) {
+		const int by = border_index(yc0 + k, h, mode);
+		bool h0 = false, h1 = false;
+		double v0 = 0.0, v1 = 0.0;
+		if (by >= 0 && bx0 >= 0)
+			v0 = input_at<BG>(fr, p.bg, has_bg, (long long)by * w + bx0, h0);
+		if (by >= 0 && bx1 >= 0)
+			v1 = input_at<BG>(fr, p.bg, has_bg, (long long)by * w + bx1, h1);
+		R.c[k][lane] = v0;
+		if (second)
+			R.c[k][64 + lane] = v1;
+		const unsigned long long m0 = __ballot(h0), m1 = __ballot(h1);
+		if (!lane) {
+			R.hole_lo[k] = m0;
+			R.hole_hi[k] = m1;
+		}
+	}
+}
+
+/* One row of S under the noise model from the region: rows holds the region's
+ * rows around it (row k + 2 of the strip is this row), med its 5x5 median of
+ * C, hm whether a sample of those rows is a hole, hc whether its own pixel
+ * is. The Laplacian's neighbours are the strip's; at a frame edge the
+ * mirror, reflect and nearest modes repeat the edge pixel for it (the
+ * median's mirror mode reflects instead, as in the region). */
+template <bool EXACT, bool BG>
+__device__ __forceinline__ void
+model_row_region(SigShared &sh, int &slot, int lane, double t_min, unsigned long long cols,
+	const Params &p, const Frame &fr, int gx, int gy, const double *rows, int k, double med, bool hm,
+	bool hc, bool own, OwnCounts &acc)
+{
+	const int mode = p.mode, w = fr.w, h = fr.h;
+	const double cc = rows[(k + 2) * 5 + 2];
+	double u = rows[(k + 1) * 5 + 2], d = rows[(k + 3) * 5 + 2];
+	double l = rows[(k + 2) * 5 + 1], r = rows[(k + 2) * 5 + 3];
+	const bool in = gx >= 0 && gx < w && gy >= 0 && gy < h;
+	if (in && mode != CUSMIC_WRAP && mode != CUSMIC_CONSTANT) {
+		u = gy - 1 < 0 ? cc : u;
+		d = gy + 1 >= h ? cc : d;
+		l = gx - 1 < 0 ? cc : l;
+		r = gx + 1 >= w ? cc : r;
+	}
+	const int mi = map_index(gx, gy, w, h, mode);
+	const double gg = fr.gain ? fr.gain[mi] : fr.g;
+	const double rn = fr.readnoise ? fr.readnoise[mi] : fr.rn;
+	const double nz = noise_model(med, gg, rn);
 	own_row<BG>(p, fr, own, (long long)gy * w + gx, cc, hc, true, acc);
 	emit_row(sh, slot, lane, t_min, cols,
-		laplacian_q<EXACT, !BG>(cc * 0.25, qu, qd, ql, qr, w == 1) / (2 * nz),
-		(in || mode == CUSMIC_WRAP) && invalid_noise(nz), hc || hn);
+		laplacian_q<EXACT, !BG>(cc * 0.25, u * 0.25, d * 0.25, l * 0.25, r * 0.25, w == 1) / (2 * nz),
+		(in || mode == CUSMIC_WRAP) && invalid_noise(nz), hm);
 }
 
 /* One wave: S of run rows [r0, r0 + NR) into the ring (run row r is frame row
@@ -1341,7 +1394,7 @@ template <bool MODEL, bool EXACT, bool BG, int NR>
 __device__ __forceinline__ bool
 significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0, int yb, int r0,
 	double t_min, unsigned long long cols, int own0, int own1, OwnCounts &acc,
-	double (&cst)[STEP_ROWS / 4])
+	double (&cst)[STEP_ROWS / 4], const ModelRegion *R, int rS0)
 {
 	constexpr bool TAME = !BG;
 	/* an opaque lane keeps the compiler from hoisting this step's per-lane
@@ -1458,30 +1511,33 @@ significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0
 			hc = hd;
 		}
 	} else {
-		/* noise model: N from strip-shared 5x5 medians, four rows at a time */
+		/* noise model: C from the step's region (loaded by the caller), N
+		 * from strip-shared 5x5 medians, four rows at a time; the region's
+		 * rows with a hole, one bit each */
+		const unsigned long long hole_rows =
+			__ballot(lane < CREG_ROWS && (R->hole_lo[lane] | R->hole_hi[lane]) != 0);
+		const int kb = r0 - rS0; /* region row of the strip's first row */
 #pragma unroll 1
 		for (int g = 0; g < nr; g += 4) {
-			const int gy0 = ya + g;
 			double rows[8 * 5], med[4];
-			bool hm = false;
 #pragma unroll
 			for (int j = 0; j < 8; ++j)
 #pragma unroll
 				for (int c = 0; c < 5; ++c)
-					rows[j * 5 + c] = med_input<BG>(fr, p.bg, has_bg, gx - 2 + c, gy0 - 2 + j, mode, hm);
+					rows[j * 5 + c] = R->c[kb + g + j][lane + c];
 			strip_median5x4(rows, med);
 			const int n_out = min(4, nr - g);
-#define MODEL_ROW(k)                                                                              \
-	model_row<EXACT, BG>(sh, slot, lane, t_min, cols, p, fr, f, gx, gy0 + (k), med[k], hm,       \
-		own_lane && gy0 + (k) >= own0 && gy0 + (k) < own1, acc)
-			MODEL_ROW(0);
-			if (n_out > 1)
-				MODEL_ROW(1);
-			if (n_out > 2)
-				MODEL_ROW(2);
-			if (n_out > 3)
-				MODEL_ROW(3);
-#undef MODEL_ROW
+#pragma unroll
+			for (int k = 0; k < 4; ++k) {
+				if (k >= n_out)
+					break;
+				const int kc = kb + g + k + HS, gy = ya + g + k;
+				const bool hm = (hole_rows >> (kb + g + k)) & 0x1full;
+				const bool hc = lane + HS < 64 ? (R->hole_lo[kc] >> (lane + HS)) & 1
+							      : (R->hole_hi[kc] >> (lane + HS - 64)) & 1;
+				model_row_region<EXACT, BG>(sh, slot, lane, t_min, cols, p, fr, gx, gy, rows, k, med[k],
+					hm, hc, own_lane && gy >= own0 && gy < own1, acc);
+			}
 		}
 	}
 	return false;
@@ -1495,7 +1551,7 @@ significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0
  * screened pixels carry SCR. */
 template <bool MODEL, bool EXACT, bool BG>
 __device__ __forceinline__ void
-significance_pass(SigShared &sh, const Params &p, int iter)
+significance_pass(SigShared &sh, ModelRegion *region, const Params &p, int iter)
 {
 	const int wave = threadIdx.x >> 6;
 	const int mode = p.mode;
@@ -1527,8 +1583,8 @@ significance_pass(SigShared &sh, const Params &p, int iter)
 		/* the run: tile column tx, tile rows [ty0, ty0 + K) of frame f */
 		const int f = p.run_frame[item];
 		const Frame fr = p.frames[f];
-		const int lr = item - fr.run0, tx = lr % fr.tiles_x, ty0 = lr / fr.tiles_x * RUN_TILES;
-		const int K = min(RUN_TILES, fr.tiles_y - ty0);
+		const int lr = item - fr.run0, tx = lr % fr.tiles_x, ty0 = lr / fr.tiles_x * p.run_tiles;
+		const int K = min(p.run_tiles, fr.tiles_y - ty0);
 		const int w = fr.w, h = fr.h, X0 = tx * TILE_W, Y0 = ty0 * TILE_H, yb = Y0 - HS;
 		const int ncols = min(TILE_W, w - X0), own1 = min(Y0 + TILE_H * K, h);
 		const unsigned long long cols = ((1ull << ncols) - 1) << HS;
@@ -1545,17 +1601,25 @@ significance_pass(SigShared &sh, const Params &p, int iter)
 			double cst[STEP_ROWS / 4];
 			bool deferred;
 			int r0, nr;
+			/* the step's first S row; under the noise model its C samples
+			 * go to the region first */
+			const int rS0 = s == 0 ? 0 : TILE_H * s + 2 * HS;
+			if (s > 0)
+				lds_barrier(); /* the last tile's medians are done with the ring */
+			if (MODEL && !fr.error) {
+				load_region<BG>(*region, p, fr, X0, yb + rS0 - HS, s == 0 ? CREG_ROWS : TILE_H + 2 * HS);
+				lds_barrier();
+			}
 			if (s == 0) {
 				r0 = wave * (STEP_ROWS / 4);
 				nr = STEP_ROWS / 4;
 				deferred = significance_rows<MODEL, EXACT, BG, STEP_ROWS / 4>(sh, p, fr, f, X0, yb,
-					r0, t_min, cols, Y0, own1, acc, cst);
+					r0, t_min, cols, Y0, own1, acc, cst, region, rS0);
 			} else {
-				lds_barrier(); /* the last tile's medians are done with the ring */
 				r0 = TILE_H * s + 2 * HS + wave * (TILE_H / 4);
 				nr = TILE_H / 4;
 				deferred = significance_rows<MODEL, EXACT, BG, TILE_H / 4>(sh, p, fr, f, X0, yb,
-					r0, t_min, cols, Y0, own1, acc, cst);
+					r0, t_min, cols, Y0, own1, acc, cst, region, rS0);
 			}
 			const int ty = Y0 + TILE_H * s; /* first frame row of the tile */
 			const int nrows = min(TILE_H, h - ty);
@@ -2007,7 +2071,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
 significance_maps(Params p, int iter)
 {
 	__shared__ SigShared sh;
-	significance_pass<false, EXACT, BG>(sh, p, iter);
+	significance_pass<false, EXACT, BG>(sh, nullptr, p, iter);
 }
 
 /* Some frame uses the noise model (5x5 medians of C in the dense part):
@@ -2017,7 +2081,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2))) void
 significance_model(Params p, int iter)
 {
 	__shared__ SigShared sh;
-	significance_pass<true, EXACT, BG>(sh, p, iter);
+	__shared__ ModelRegion region;
+	significance_pass<true, EXACT, BG>(sh, &region, p, iter);
 }
 
 __device__ __forceinline__ double
