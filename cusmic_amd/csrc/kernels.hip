@@ -1056,6 +1056,42 @@ static_assert(SW == 64, "the significance region is one wave wide");
 static_assert(STEP_ROWS % 4 == 0 && TILE_H % 4 == 0, "four waves share the rows of a step");
 static_assert(RING >= STEP_ROWS && RING <= 64, "the ring holds the rows of one tile");
 
+/* The 5x5 median of S, v the window in row-major order. Without a NaN every
+ * correct selection gives the same value, so the min/max network does. A NaN
+ * (S made from infinite samples) stops the reference's compare-exchange
+ * (src/sort.cuh:18-36: odd-even transposition passes that swap only when
+ * b < a), so its result depends on where the NaN lies; such a window takes
+ * that sort. */
+__device__ __forceinline__ double
+s_median25(const double *v)
+{
+	bool nan = false;
+#pragma unroll
+	for (int k = 0; k < 25; ++k)
+		nan |= isnan(v[k]);
+	double m;
+	if (!nan) {
+		median25(v, &m);
+		return m;
+	}
+	double s[25];
+#pragma unroll
+	for (int k = 0; k < 25; ++k)
+		s[k] = v[k];
+#pragma unroll
+	for (int pass = 0; pass < 13; ++pass)
+#pragma unroll
+		for (int start = 0; start < 2; ++start)
+#pragma unroll
+			for (int i = start; i + 1 < 25; i += 2) {
+				const double a = s[i], b = s[i + 1];
+				const bool swap = b < a;
+				s[i] = swap ? b : a;
+				s[i + 1] = swap ? a : b;
+			}
+	return s[12];
+}
+
 /* max(v, 0) as the reference writes it (v < 0 ? 0 : v), which keeps a NaN.
  * Without infinite samples (TAME) no NaN can arise and one max instruction
  * does; the two differ at most in the sign of a zero, which no threshold test
@@ -1378,12 +1414,10 @@ model_row_region(SigShared &sh, int &slot, int lane, double t_min, unsigned long
 /* One wave: S of run rows [r0, r0 + NR) into the ring (run row r is frame row
  * yb + r, lane l is column X0 - HS + l), with each row's screened columns
  * (among cols), whether it holds an invalid noise value and whether a sample
- * of C it read is a hole. C is made from the input here; the rows
- * [own0, own1) and the columns of cols belong to this run (own_row). The
- * interior path leaves their C in cst and returns true: a store issued
- * between the loads would hold up every later load's wait (loads and stores
- * share one counter), so the caller stores the rows after the step's loads.
- * The other paths store C at once. Without a background no sample of C is infinite,
+ * of C it read is a hole. C is made from the input here and stored at once
+ * for the run's own pixels: the rows [own0, own1) and the columns of cols
+ * (own_row). Storing inside the row loop keeps no C in registers, which
+ * leaves them to loads five rows ahead. Without a background no sample of C is infinite,
  * so the Laplacian clips with one max instruction; with one the reference's
  * NaN-keeping clip is used. Slots outside the frame are computed from
  * in-bounds samples; the caller overwrites the ones the medians read, unless
@@ -1391,10 +1425,10 @@ model_row_region(SigShared &sh, int &slot, int lane, double t_min, unsigned long
  * extension. NR is a constant so that the row loop unrolls around its ballots
  * and the loads of several rows are in flight. */
 template <bool MODEL, bool EXACT, bool BG, int NR>
-__device__ __forceinline__ bool
+__device__ __forceinline__ void
 significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0, int yb, int r0,
 	double t_min, unsigned long long cols, int own0, int own1, OwnCounts &acc,
-	double (&cst)[STEP_ROWS / 4], const ModelRegion *R, int rS0)
+	const ModelRegion *R, int rS0)
 {
 	constexpr bool TAME = !BG;
 	/* an opaque lane keeps the compiler from hoisting this step's per-lane
@@ -1421,7 +1455,7 @@ significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0
 		 * centres (a wave shift). Lanes 0 and 63 take theirs from two scalar
 		 * loads per row (the address is the wave's; the input is read-only
 		 * here), which leaves one vector load fewer per row and the registers
-		 * for a third row in flight. This is the error-map kernel without a
+		 * for more rows in flight. This is the error-map kernel without a
 		 * background (SEDGE); elsewhere the edge samples are vector loads:
 		 * lanes 0 and 63 load theirs, the others reload their centre in the
 		 * same instruction, two rows ahead. */
@@ -1432,7 +1466,7 @@ significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0
 		cdouble *dl = (cdouble *)lb, *dr = (cdouble *)(lb + 8ull * (SW + 1));
 		/* the samples of row r: the pixel below, the edge samples and the
 		 * noise (and their background), fetched PF rows ahead */
-		constexpr int PF = SEDGE ? 3 : 2;
+		constexpr int PF = SEDGE ? 5 : 2;
 		double dn[NR], de[NR], el[NR], er[NR], nz[NR], bn[NR], be[NR];
 #define FETCH_ROW(k)                                                                       \
 	do {                                                                               \
@@ -1459,8 +1493,7 @@ significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0
 		for (int r = 0; r < PF && r < NR; ++r)
 			FETCH_ROW(r);
 		/* fully unrolled and branch-free (constant NR, convergent ballots):
-		 * the loads run PF rows ahead, and the rows' C stays in registers for
-		 * the caller to store after the step's loads */
+		 * the loads run PF rows ahead */
 #pragma unroll
 		for (int r = 0; r < NR; ++r) {
 			if (r + PF < NR)
@@ -1474,8 +1507,13 @@ significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0
 			const double ql = wave_shift_right(qc, qe), qr = wave_shift_left(qc, qe);
 			const double qd = cd * 0.25;
 			const int gy = ya + r;
-			own_row<BG>(p, fr, own_lane && gy >= own0 && gy < own1, o + r * W, cc, hc, false, acc);
-			cst[r] = cc;
+			/* non-temporal: C is read again only by later kernels, so it
+			 * should not displace the input lines that neighbouring tiles read
+			 * from L2 */
+			const bool own = own_lane && gy >= own0 && gy < own1;
+			if (own)
+				__builtin_nontemporal_store(cc, &p.clean[fr.off + o + r * W]);
+			own_row<BG>(p, fr, own, o + r * W, cc, hc, false, acc);
 			emit_row(sh, slot, lane, t_min, cols,
 				laplacian_q<EXACT, TAME>(qc, qu, qd, ql, qr, false) / (2 * nz[r]),
 				invalid_noise(nz[r]), hu || hc || hd || hs);
@@ -1486,7 +1524,6 @@ significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0
 			hc = hd;
 		}
 #undef FETCH_ROW
-		return true;
 	} else if (!MODEL || fr.error) {
 		bool hu = false, hc = false;
 		double qu = lap_input<BG>(fr, p.bg, has_bg, gx, ya - 1, mode, hu) * 0.25;
@@ -1540,7 +1577,6 @@ significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0
 			}
 		}
 	}
-	return false;
 }
 
 /* The first detection pass, which also does the prologue's work (C, flags,
@@ -1598,9 +1634,6 @@ significance_pass(SigShared &sh, ModelRegion *region, const Params &p, int iter)
 			const int lane = tid & 63;
 			/* 1. S rows: the first tile needs rows [0, STEP_ROWS) of the run,
 			 * each later one TILE_H more */
-			double cst[STEP_ROWS / 4];
-			bool deferred;
-			int r0, nr;
 			/* the step's first S row; under the noise model its C samples
 			 * go to the region first */
 			const int rS0 = s == 0 ? 0 : TILE_H * s + 2 * HS;
@@ -1611,15 +1644,13 @@ significance_pass(SigShared &sh, ModelRegion *region, const Params &p, int iter)
 				lds_barrier();
 			}
 			if (s == 0) {
-				r0 = wave * (STEP_ROWS / 4);
-				nr = STEP_ROWS / 4;
-				deferred = significance_rows<MODEL, EXACT, BG, STEP_ROWS / 4>(sh, p, fr, f, X0, yb,
-					r0, t_min, cols, Y0, own1, acc, cst, region, rS0);
+				const int r0 = wave * (STEP_ROWS / 4);
+				significance_rows<MODEL, EXACT, BG, STEP_ROWS / 4>(sh, p, fr, f, X0, yb, r0, t_min,
+					cols, Y0, own1, acc, region, rS0);
 			} else {
-				r0 = TILE_H * s + 2 * HS + wave * (TILE_H / 4);
-				nr = TILE_H / 4;
-				deferred = significance_rows<MODEL, EXACT, BG, TILE_H / 4>(sh, p, fr, f, X0, yb,
-					r0, t_min, cols, Y0, own1, acc, cst, region, rS0);
+				const int r0 = TILE_H * s + 2 * HS + wave * (TILE_H / 4);
+				significance_rows<MODEL, EXACT, BG, TILE_H / 4>(sh, p, fr, f, X0, yb, r0, t_min,
+					cols, Y0, own1, acc, region, rS0);
 			}
 			const int ty = Y0 + TILE_H * s; /* first frame row of the tile */
 			const int nrows = min(TILE_H, h - ty);
@@ -1713,20 +1744,6 @@ significance_pass(SigShared &sh, ModelRegion *region, const Params &p, int iter)
 			}
 			lds_barrier();
 
-			/* the deferred C of this wave's rows: its latency overlaps the
-			 * medians below. Non-temporal: C is read again only by later
-			 * kernels, so it should not displace the input lines that
-			 * neighbouring tiles read from L2. */
-			if (deferred) {
-				const int ya = yb + r0;
-				double *cp = p.clean + fr.off + (long long)ya * w + X0 - HS + lane;
-				const bool own_lane = (cols >> lane) & 1;
-#pragma unroll
-				for (int r = 0; r < STEP_ROWS / 4; ++r)
-					if (r < nr && own_lane && ya + r >= Y0 && ya + r < own1)
-						__builtin_nontemporal_store(cst[r], &cp[(long long)r * w]);
-			}
-
 			/* 4. S' at the screened pixels, which get SCR (no hole lies in the
 			 * tile); those above t are buffered as seeds (a batch adds at most
 			 * BLOCK, and the buffer keeps room for that many). Excluded pixels
@@ -1740,7 +1757,7 @@ significance_pass(SigShared &sh, ModelRegion *region, const Params &p, int iter)
 					int pos = sh.alist[idx], row = pos >> 6, c = pos & 63;
 					i = (ty + row) * w + X0 - HS + c;
 					int rc = r_lo + HS + row;
-					double v[25], m;
+					double v[25];
 #pragma unroll
 					for (int dy = 0; dy < 5; ++dy) {
 						int sl = (rc - 2 + dy) % RING;
@@ -1748,8 +1765,7 @@ significance_pass(SigShared &sh, ModelRegion *region, const Params &p, int iter)
 						for (int dx = 0; dx < 5; ++dx)
 							v[dy * 5 + dx] = sh.s[sl][c - 2 + dx];
 					}
-					median25(v, &m);
-					double sp = v[12] - m;
+					double sp = v[12] - s_median25(v);
 					p.sprime[fr.off + i] = sp;
 					p.flags[fr.off + i] = uint8_t(SCR | (fr.mask && fr.mask[i] ? EXCL : 0u));
 					seed = sp > t_cr;
@@ -2010,12 +2026,11 @@ significance_blocks_pass(BlockShared &sh, const Params &p, int iter)
 			const long long g = fr.off + (long long)y * w + x;
 			const bool screened = unsafe || S[c] > t_min;
 			if (screened) {
-				double v[25], m;
+				double v[25];
 #pragma unroll
 				for (int k = 0; k < 25; ++k)
 					v[k] = S[c + (k / 5 - 2) * BR + k % 5 - 2];
-				median25(v, &m);
-				double sp = v[12] - m;
+				double sp = v[12] - s_median25(v);
 				p.sprime[g] = sp;
 				seed = sp > t_cr;
 			}

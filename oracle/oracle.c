@@ -64,9 +64,12 @@ oracle_border_index(long x, long n, int mode)
 	return mode == CUSMIC_REFLECT ? p - 1 - x : p - x;
 }
 
-/* Median of the size x size window with scipy border semantics; a median is
- * an element selection, so any correct selection gives the same bits
- * (src/filters.cuh:46-70). */
+/* Median of the size x size window with scipy border semantics, the window
+ * in row-major order (src/filters.cuh:46-70). The reference sorts it with
+ * odd-even transposition passes that swap only when b < a (src/sort.cuh:18-36);
+ * the insertion sort below gives the same array: neither moves a NaN, and each
+ * run of values between NaNs ends sorted in place, so a window with a NaN (S
+ * made from infinite samples) has a median that depends on the positions. */
 void
 oracle_median(const double *in, double *out, long w, long h, int size, int mode)
 {

@@ -511,3 +511,28 @@ def test_iteration_log(cuda, caplog):
         flagged = now
     assert lines == want, (lines, want)
     assert int(mask.sum()) == flagged and len(want) > 2
+
+
+def test_background_overflow(cuda):
+    """src/cleaner.cuh:87-97: a background that makes every donor infinite
+    while holes need filling raises; one that overflows only some pixels
+    (infinite samples of C) gives the oracle's masks and pixels."""
+    say("background overflow: no finite donor")
+    d = np.full((20, 24), 1.5e308)
+    d[5, 7] = np.nan
+    compare([dict(data=d, error=np.ones_like(d), background=np.full_like(d, 1e308))])
+    with pytest.raises(ValueError, match="no finite replacement donors"):
+        gpu_clean([dict(data=d, error=np.ones_like(d), background=np.full_like(d, 1e308))])
+    say("background overflow at some pixels")
+    rng = np.random.default_rng(17)
+    d, err, m = scene((96, 112), 23, nan=2e-3, mask_frac=2e-3)
+    big = rng.random(d.shape) < 0.01
+    d = d.copy()
+    d[big] = 1.5e308
+    bg = np.where(big, 1e308, rng.uniform(-20, 20, d.shape))
+    for mode in ("mirror", "constant"):
+        got, want = compare([dict(data=d, error=err, mask=m, background=bg)], border_mode=mode,
+                            record_donors=True)
+        assert want[1][0].sum() > 10
+    say("background overflow, noise model")
+    compare([dict(data=d, mask=m, background=bg)], frame_gain=[2.0], frame_readnoise=[5.0])

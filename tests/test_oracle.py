@@ -46,6 +46,53 @@ def test_stage_golden_vectors():
     assert checked > 500
 
 
+def transposition_sorted(values):
+    """The reference's window sort (src/sort.cuh:18-36), element by element:
+    odd-even transposition passes that swap a pair only when b < a."""
+    v = list(values)
+    for _ in range((len(v) + 1) // 2):
+        for start in (0, 1):
+            for i in range(start, len(v) - 1, 2):
+                if v[i + 1] < v[i]:
+                    v[i], v[i + 1] = v[i + 1], v[i]
+    return v
+
+
+def segment_median(values):
+    """The same result in closed form: no compare with a NaN swaps, so a NaN
+    stays where it is, and each run between NaNs (at most 25 long, 26 phases)
+    ends sorted in place; the median is the middle slot."""
+    v = np.asarray(values)
+    mid = len(v) // 2
+    if np.isnan(v[mid]):
+        return np.nan
+    nan = np.flatnonzero(np.isnan(v))
+    lo = nan[nan < mid].max() + 1 if (nan < mid).any() else 0
+    hi = nan[nan > mid].min() if (nan > mid).any() else len(v)
+    return np.sort(v[lo:hi])[mid - lo]
+
+
+def test_median_with_nan_follows_the_reference_sort():
+    """5x5 windows holding NaN (S of infinite samples): the median is the
+    middle slot of the reference's sorted window, which depends on where the
+    NaN lies, and not the median of the finite values."""
+    rng = np.random.default_rng(5)
+    checked = differs = 0
+    for trial in range(400):
+        window = rng.integers(0, 6, 25).astype(np.float64)
+        window[rng.random(25) < rng.choice([0.04, 0.15, 0.4])] = np.nan
+        if not np.isnan(window).any():
+            window[rng.integers(25)] = np.nan
+        got = oracle.median(window.reshape(5, 5), 5, "nearest")[2, 2]
+        want = transposition_sorted(window)[12]
+        np.testing.assert_array_equal(got, want, err_msg=str(window))
+        np.testing.assert_array_equal(got, segment_median(window))
+        finite = window[~np.isnan(window)]
+        differs += bool(np.isnan(got) or got != np.sort(finite)[len(finite) // 2])
+        checked += 1
+    assert checked == 400 and differs > 100
+
+
 @pytest.mark.parametrize("mode", ["mirror", "reflect", "nearest", "wrap", "constant"])
 def test_laplacian_kat(mode):
     """test/test_cleaner.py:33-47 on both orders: integer frames where the two
