@@ -1,22 +1,30 @@
 #!/usr/bin/env python3
 """Benchmark of the L.A.Cosmic cleaning loop on one MI355X.
 
-Workload (default): the ragged batch -- 64 float64 frames with shapes drawn
-from {256x256, 512x2048, 1024x1024, 2048x4096, 4096x4096}, per-frame seeds and
-error maps, 2e-4 cosmic-ray tracks/px, settings (3, 5, 3), maxiter 4, mirror
-borders -- cleaned in one call of remove_cosmics_ragged_device with inputs
-resident in HBM. A step is one such call. Also reported:
-  * per_frame: the same 64 frames as 64 separate plain-path calls;
+Workload (the headline): the ragged batch -- 64 float64 frames with shapes
+drawn from {256x256, 512x2048, 1024x1024, 2048x4096, 4096x4096}, per-frame
+seeds and error maps, 2e-4 cosmic-ray tracks/px, settings (3, 5, 3), maxiter
+4, mirror borders -- cleaned in one call of remove_cosmics_ragged_device with
+inputs resident in HBM. A step is one such call. Also reported, each outside
+the timed steps:
+  * per_frame: the same 64 frames as 64 separate plain-path calls, whose
+    outputs must equal the ragged call's bit for bit;
+  * detector: one 4096x4096 frame with an error map, background 150 and 0.5%
+    masked pixels (the detector-size configuration), with the roofline of its
+    first pass;
+  * host_call / host_call_512: the host entry point remove_cosmics on the
+    batch's largest frame, and on the reference's saved 512x512 scene with
+    its settings (1, 5, 5) -- complete calls, as the reference times them;
   * fwd_bwd: 8 x 1024^2 noise-model frames, forward with donor recording then
-    the adjoint of loss = sum(w * cleaned), vs forward alone;
-  * roofline: the dense significance kernel's algorithmic bytes (8 B of clean
-    image + 8 B of error map per processed pixel) over its event-timed
-    duration; call_roofline: the whole call's algorithmic bytes
-    (17 + M + I_d (8 + E) + I_r 8 per pixel) over the step time;
-  * cpu_baseline: the oracle (the C restatement, one thread) on a bounded
-    sample of the same workload; cpu_baseline_all_cores: the same oracle on
-    every host core (OpenMP over rows) on up to four 1 Mpx frames.
-Prints one JSON line on stdout; progress goes to stderr.
+    the adjoint of loss = sum(w * cleaned), against the forward alone;
+  * roofline: the dominant kernel (the first detection pass) -- its
+    algorithmic bytes over its hipEvent time; call_roofline: the whole
+    call's algorithmic bytes (17 + M + I_d (8 + E) + I_r 8 per pixel);
+  * cpu_baseline: the oracle (the C restatement, one thread) on one 1 Mpx frame
+    of the batch, checked against the GPU's output for that frame;
+    cpu_baseline_all_cores: the same oracle on every host core on four frames.
+Prints one JSON line on stdout; progress goes to stderr. ``--only LEG``
+(detector or fwd-bwd) runs one leg alone, for profiling.
 """
 
 import argparse
@@ -32,9 +40,11 @@ sys.path.insert(0, str(ROOT))
 METRIC = "cleaned Mpix/s fp64 maxiter=4: ragged-batch vs per-frame; fwd+bwd Mpix/s"
 HBM_PEAK_GBS = 8000.0
 # first detection pass, per frame pixel: reads the data and the error map,
-# writes the working image and the flags
+# writes the working image and the flags (plus the input mask where there is
+# one)
 FIRST_BYTES_PER_PX = 8.0 + 8.0 + 8.0 + 1.0
 TRAFFIC_FILE = ROOT / "profiles" / "significance_maps_ragged64_traffic.json"
+SAVED_SCENE = ROOT / "tests" / "golden" / "fits"
 
 
 def significance_traffic(launches):
@@ -70,130 +80,113 @@ def cpu_info():
     return cores, model
 
 
-def main():
-    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--frames", type=int, default=64)
-    ap.add_argument("--no-per-frame", action="store_true")
-    ap.add_argument("--no-backward", action="store_true")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-host", action="store_true")
-    ap.add_argument("--calibration-copy", action="store_true",
-                    help="start with a 1 GiB device-to-device copy, whose hardware counter "
-                         "readings scale FETCH_SIZE/WRITE_SIZE to bytes")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU only; on several GPUs the frames of a batch would be "
-              "sharded by pixel count, with no exchange between GPUs", file=sys.stderr)
-        sys.exit(2)
+def roofline(bytes_per_launch, ms, launches=1, traffic=None, source=None, kernel="significance_maps"):
+    achieved = bytes_per_launch / (ms / 1e3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kernel,
+            "algorithmic_bytes_per_launch": round(bytes_per_launch), "avg_launch_ms": round(ms, 4),
+            "launches": launches, "traffic_source": source}
 
-    import numpy as np
-    import torch
 
-    from cusmic_amd import _lib, engine, scenes
+class Bench:
+    def __init__(self, args):
+        import numpy as np
+        import torch
 
-    dev = torch.device("cuda", 0)
-    _lib.lib()
-    if args.calibration_copy:
-        a = torch.ones(1 << 27, dtype=torch.float64, device=dev)
-        torch.empty_like(a).copy_(a)
+        from cusmic_amd import _lib, engine, scenes
+
+        self.args, self.np, self.torch = args, np, torch
+        self.lib, self.engine, self.scenes = _lib, engine, scenes
+        self.dev = torch.device("cuda", 0)
+        _lib.lib()
+
+    def run(self, fr, clean=None, mask=None, profile=None, **kw):
+        return self.engine.run(fr, self.dev, 3.0, 5.0, 3.0, 4, self.lib.MIRROR, clean=clean, mask=mask,
+                               profile=profile, **kw)
+
+    # ------------------------------------------------------------ ragged
+    def ragged(self):
+        np, torch, args = self.np, self.torch, self.args
+        log("generating the ragged batch on the GPU")
+        self.batch = batch = self.scenes.ragged_batch(self.dev, n=args.frames)
+        fr = self.engine.Frames()
+        for item in batch:
+            fr.add(item["data"], error=item["error"])
+        total = self.total = fr.total
+        self.mpix = mpix = total / 1e6
+        self.clean = clean = torch.empty(total, dtype=torch.float64, device=self.dev)
+        self.mask = mask = torch.empty(total, dtype=torch.uint8, device=self.dev)
+        log(f"{len(batch)} frames, {mpix:.1f} Mpx")
+        for i in range(args.warmup):
+            self.run(fr, clean, mask)
+            log(f"warmup {i + 1}/{args.warmup}")
+        profiles = []
         torch.cuda.synchronize()
-        del a
-    log("generating the ragged batch on the GPU")
-    batch = scenes.ragged_batch(dev, n=args.frames)
-    fr = engine.Frames()
-    for item in batch:
-        fr.add(item["data"], error=item["error"])
-    total = fr.total
-    mpix = total / 1e6
-    clean = torch.empty(total, dtype=torch.float64, device=dev)
-    mask = torch.empty(total, dtype=torch.uint8, device=dev)
-    log(f"{len(batch)} frames, {mpix:.1f} Mpx")
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            prof = {}
+            self.run(fr, clean, mask, prof)
+            profiles.append(prof)
+        torch.cuda.synchronize()
+        self.ms_step = ms_step = 1e3 * (time.perf_counter() - t0) / args.steps
+        value = mpix / (ms_step / 1e3)
+        log(f"ragged: {ms_step:.3f} ms/step, {value:.1f} Mpix/s")
 
-    def step(profile=None):
-        return engine.run(fr, dev, 3.0, 5.0, 3.0, 4, _lib.MIRROR, clean=clean, mask=mask, profile=profile)
+        def mean(key):
+            return float(np.mean([p[key] for p in profiles]))
 
-    for i in range(args.warmup):
-        step()
-        log(f"warmup {i + 1}/{args.warmup}")
-    profiles = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        prof = {}
-        step(prof)
-        profiles.append(prof)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms_step = 1e3 * elapsed / args.steps
-    value = mpix / (ms_step / 1e3)
-    log(f"ragged: {ms_step:.3f} ms/step, {value:.1f} Mpix/s")
-    flagged = int(mask.sum())
+        # the dominant kernel: the first detection pass (one launch per call),
+        # timed with hipEvents on its stream
+        sig_ms = mean("first_pass_ms")
+        traffic, source = significance_traffic(1)
+        # the whole call, the dense loop's algorithmic bytes per frame pixel:
+        # B = 17 + M + I_d (8 + E) + I_r 8 with M = 8 (error map read once) and
+        # E = 8 (its re-read per detection pass); I_d / I_r are the passes each
+        # frame ran. The incremental passes touch only the pixels near changes,
+        # so the figure can exceed 100%.
+        call_bytes = 25.0 * total + 16.0 * mean("detect_pass_pixels") + 8.0 * mean("replace_pass_pixels")
+        call_gbs = call_bytes / (ms_step / 1e3) / 1e9
+        return {
+            "metric": METRIC, "value": round(value, 2), "unit": "Mpix/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": f"ragged batch of {len(batch)} float64 frames (256^2..4096^2), "
+                                   "per-frame error maps, 2e-4 CR tracks/px, contrast 3, thresholds 5/3, "
+                                   "maxiter 4, mirror, one remove_cosmics_ragged_device call",
+                       "frames": len(batch), "megapixels": round(mpix, 3), "maxiter": 4,
+                       "flagged_pixels": int(mask.sum()),
+                       "detection_passes_per_pixel": round(mean("detect_pass_pixels") / total, 4),
+                       "replacement_passes_per_pixel": round(mean("replace_pass_pixels") / total, 4)},
+            "roofline": roofline(FIRST_BYTES_PER_PX * total, sig_ms, 1, traffic, source),
+            "call_roofline": {"bytes_per_pixel": round(call_bytes / total, 3), "achieved": round(call_gbs, 1),
+                              "unit": "GB/s", "frac": round(call_gbs / HBM_PEAK_GBS, 4)},
+            "stages_ms": {k: round(mean(k), 4) for k in
+                          ("prologue_ms", "first_pass_ms", "significance_ms", "detect_ms", "replace_ms",
+                           "epilogue_ms")},
+            "lists_per_call": {k: int(profiles[0][k]) for k in ("seeds", "candidates", "grown", "widened")},
+            "block_pass_pixels": int(mean("detect_pixels") - total),
+        }
 
-    def mean(key):
-        return float(np.mean([p[key] for p in profiles]))
-
-    # dominant kernel: the dense first detection pass (significance_maps, one
-    # launch per call) over every frame pixel, timed with hipEvents on its
-    # stream
-    sig_ms, launches = mean("first_pass_ms"), 1
-    sig_bytes = FIRST_BYTES_PER_PX * total
-    achieved = sig_bytes / (sig_ms / 1e3) / 1e9
-    traffic, traffic_note = significance_traffic(launches)
-    # whole call, the dense loop's algorithmic bytes per frame pixel:
-    # B = 17 + M + I_d (8 + E) + I_r 8 with M = 8 (error map read once) and
-    # E = 8 (its re-read per detection pass); I_d / I_r are the detection and
-    # replacement passes each frame ran. The incremental passes touch only
-    # the pixels near changes, so the figure can exceed 100%.
-    call_bytes = 25.0 * total + 16.0 * mean("detect_pass_pixels") + 8.0 * mean("replace_pass_pixels")
-    call_gbs = call_bytes / (ms_step / 1e3) / 1e9
-    result = {
-        "metric": METRIC, "value": round(value, 2), "unit": "Mpix/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": f"ragged batch of {len(batch)} float64 frames (256^2..4096^2), "
-                               "per-frame error maps, 2e-4 CR tracks/px, contrast 3, thresholds 5/3, "
-                               "maxiter 4, mirror, one remove_cosmics_ragged_device call",
-                   "frames": len(batch), "megapixels": round(mpix, 3), "maxiter": 4,
-                   "flagged_pixels": flagged,
-                   "detection_passes_per_pixel": round(mean("detect_pass_pixels") / total, 4),
-                   "replacement_passes_per_pixel": round(mean("replace_pass_pixels") / total, 4)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "kernel": "significance_maps",
-                     "algorithmic_bytes_per_launch": round(sig_bytes / launches),
-                     "avg_launch_ms": round(sig_ms / launches, 4), "launches": launches,
-                     "traffic_source": traffic_note},
-        "call_roofline": {"bytes_per_pixel": round(call_bytes / total, 3),
-                          "achieved": round(call_gbs, 1), "unit": "GB/s",
-                          "frac": round(call_gbs / HBM_PEAK_GBS, 4)},
-        "stages_ms": {k: round(mean(k), 4) for k in
-                      ("prologue_ms", "first_pass_ms", "significance_ms", "detect_ms", "replace_ms",
-                       "epilogue_ms")},
-        "lists_per_call": {k: int(profiles[0][k]) for k in ("seeds", "candidates", "grown", "widened")},
-        "block_pass_pixels": int(mean("detect_pixels") - total),
-    }
-
-    if not args.no_per_frame:
+    # --------------------------------------------------------- per frame
+    def per_frame(self):
+        torch = self.torch
         log("per-frame plain path")
         singles = []
-        for item in batch:
-            f1 = engine.Frames()
+        for item in self.batch:
+            f1 = self.engine.Frames()
             f1.add(item["data"], error=item["error"])
             singles.append(f1)
 
-        def per_frame():
+        def all_frames():
             for f1 in singles:
-                engine.run(f1, dev, 3.0, 5.0, 3.0, 4, _lib.MIRROR)
-        per_frame()
+                self.run(f1)
+        all_frames()
         torch.cuda.synchronize()
-        reps = max(1, args.steps // 4)
+        reps = max(1, self.args.steps // 4)
         t0 = time.perf_counter()
         for _ in range(reps):
-            per_frame()
+            all_frames()
         torch.cuda.synchronize()
         pf_ms = 1e3 * (time.perf_counter() - t0) / reps
         # outside the timed region: every frame's plain-path output equals its
@@ -201,84 +194,122 @@ def main():
         same, off = 0, 0
         for f1 in singles:
             n1 = f1.total
-            c1, m1, _, _ = engine.run(f1, dev, 3.0, 5.0, 3.0, 4, _lib.MIRROR)
-            same += bool(torch.equal(c1.view(torch.int64), clean[off:off + n1].view(torch.int64))
-                         and torch.equal(m1, mask[off:off + n1]))
+            c1, m1, _, _ = self.run(f1)
+            same += bool(torch.equal(c1.view(torch.int64), self.clean[off:off + n1].view(torch.int64))
+                         and torch.equal(m1, self.mask[off:off + n1]))
             off += n1
-        result["per_frame"] = {"value": round(mpix / (pf_ms / 1e3), 2), "unit": "Mpix/s",
-                               "ms_per_batch": round(pf_ms, 3),
-                               "ragged_speedup": round(pf_ms / ms_step, 3),
-                               "outputs_equal_ragged": same == len(singles),
-                               "frames_equal": same}
         log(f"per-frame: {pf_ms:.3f} ms/batch; {same}/{len(singles)} frames equal the ragged output")
-        del singles
+        return {"value": round(self.mpix / (pf_ms / 1e3), 2), "unit": "Mpix/s", "ms_per_batch": round(pf_ms, 3),
+                "ragged_speedup": round(pf_ms / self.ms_step, 3), "outputs_equal_ragged": same == len(singles),
+                "frames_equal": same}
 
-    if not args.no_host:
-        # the host entry point on the batch's largest frame: validation,
-        # allocation and pageable copies both ways included (never `value`)
+    # ---------------------------------------------------------- detector
+    def detector(self):
+        np, torch = self.np, self.torch
+        log("detector frame: 4096x4096, error map, background 150, 0.5% masked")
+        sc = self.scenes.detector_frame(self.dev)
+        fr = self.engine.Frames()
+        fr.add(sc["data"], error=sc["error"], mask=sc["mask"])
+        n = fr.total
+        clean = torch.empty(n, dtype=torch.float64, device=self.dev)
+        mask = torch.empty(n, dtype=torch.uint8, device=self.dev)
+        for _ in range(3):
+            self.run(fr, clean, mask, background=sc["background"])
+        profiles, samples = [], []
+        for _ in range(max(5, self.args.steps)):
+            prof = {}
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            self.run(fr, clean, mask, prof, background=sc["background"])
+            torch.cuda.synchronize()
+            samples.append(1e3 * (time.perf_counter() - t0))
+            profiles.append(prof)
+        ms = float(np.median(samples))
+        first = float(np.median([p["first_pass_ms"] for p in profiles]))
+        log(f"detector: {ms:.3f} ms per call, first pass {first:.3f} ms")
+        # first pass per pixel: data, error map, working image, flags and the
+        # input mask (the scalar background costs no bytes)
+        return {"value": round(n / 1e6 / (ms / 1e3), 2), "unit": "Mpix/s", "ms_per_call": round(ms, 4),
+                "frame": "4096x4096, error map, background 150, 0.5% masked", "flagged_pixels": int(mask.sum()),
+                "first_pass_roofline": roofline((FIRST_BYTES_PER_PX + 1.0) * n, first)}
+
+    # -------------------------------------------------------------- host
+    def host(self):
         import ctypes
-        big = max(batch, key=lambda it: it["data"].numel())
+
+        np, lib = self.np, self.lib
+        from cusmic_amd import fits
+
+        def host_call(img, opts, out, flags):
+            msg = ctypes.create_string_buffer(256)
+            lib.check(lib.lib().remove_cosmics(ctypes.byref(img), ctypes.byref(opts), out.ctypes.data,
+                                               flags.ctypes.data, msg, 256), msg)
+
+        result = {}
+        # the host entry point on the batch's largest frame: validation,
+        # allocation and copies both ways included (never `value`)
+        big = max(self.batch, key=lambda it: it["data"].numel())
         d = np.ascontiguousarray(big["data"].cpu().numpy())
         e = np.ascontiguousarray(big["error"].cpu().numpy())
         hh, ww = d.shape
-        out_h = np.empty_like(d)
-        mask_h = np.empty(d.shape, np.uint8)
-        img = _lib.CusmicImage(d.ctypes.data, e.ctypes.data, None, None, None, None, ww, hh)
-        opts = _lib.default_options()
-        msg = ctypes.create_string_buffer(256)
-
-        def host_call():
-            _lib.check(_lib.lib().remove_cosmics(ctypes.byref(img), ctypes.byref(opts), out_h.ctypes.data,
-                                                 mask_h.ctypes.data, msg, 256), msg)
+        out, flags = np.empty_like(d), np.empty(d.shape, np.uint8)
+        img = lib.CusmicImage(d.ctypes.data, e.ctypes.data, None, None, None, None, ww, hh)
+        opts = lib.default_options()
         log(f"host entry point, {hh}x{ww}")
-        host_call()
+        host_call(img, opts, out, flags)
         t0 = time.perf_counter()
-        reps = 3
-        for _ in range(reps):
-            host_call()
-        host_ms = 1e3 * (time.perf_counter() - t0) / reps
-        result["host_call"] = {"value": round(d.size / 1e6 / (host_ms / 1e3), 2), "unit": "Mpix/s",
-                               "ms_per_call": round(host_ms, 3), "frame": f"{hh}x{ww}, error map",
-                               "note": "remove_cosmics on host arrays: validation, allocation and "
-                                       "pageable copies to and from the device included"}
-        log(f"host call: {host_ms:.3f} ms")
-        del d, e, out_h, mask_h
-
-        # complete calls on one 512x512 frame with an error map of 5 and
-        # settings (1, 5, 5), maxiter 4, mirror: the shape and settings of the
-        # reference's published complete-call runs (README.md:123-140), on a
-        # synthetic scene; 4 warmups, median of 16 (bench/bench.cu:204-257)
-        d, _ = scenes.frame((512, 512), 11, "cpu")
-        d = np.ascontiguousarray(d.numpy())
-        e = np.full_like(d, 5.0)
-        out_h = np.empty_like(d)
-        mask_h = np.empty(d.shape, np.uint8)
-        img = _lib.CusmicImage(d.ctypes.data, e.ctypes.data, None, None, None, None, 512, 512)
-        opts = _lib.default_options()
+        for _ in range(3):
+            host_call(img, opts, out, flags)
+        ms = 1e3 * (time.perf_counter() - t0) / 3
+        result["host_call"] = {"value": round(d.size / 1e6 / (ms / 1e3), 2), "unit": "Mpix/s",
+                               "ms_per_call": round(ms, 3), "frame": f"{hh}x{ww}, error map",
+                               "note": "remove_cosmics on host arrays: validation, allocation and copies "
+                                       "to and from the device included"}
+        log(f"host call: {ms:.3f} ms")
+        # complete calls on the reference's saved 512x512 scene, error map 5,
+        # settings (1, 5, 5), maxiter 4, mirror: the inputs of its only
+        # published timings; first call apart, then 4 warmups and the median
+        # of 16 (bench/bench.cu:204-257); output checked outside the timing
+        d = np.ascontiguousarray(fits.read_fits(SAVED_SCENE / "input.fits.gz", dtype="float64")[0])
+        e = np.ascontiguousarray(fits.read_fits(SAVED_SCENE / "error.fits.gz", dtype="float64")[0])
+        ref = fits.read_fits(SAVED_SCENE / "reference.fits.gz")[0]
+        crmask = fits.read_fits(SAVED_SCENE / "reference.fits.gz", ext="CRMASK")[0].astype(bool)
+        out, flags = np.empty_like(d), np.empty(d.shape, np.uint8)
+        img = lib.CusmicImage(d.ctypes.data, e.ctypes.data, None, None, None, None, 512, 512)
+        opts = lib.default_options()
         opts.contrast, opts.cr_threshold, opts.neighbor_threshold = 1.0, 5.0, 5.0
+        t0 = time.perf_counter()
+        host_call(img, opts, out, flags)
+        first = 1e3 * (time.perf_counter() - t0)
         for _ in range(4):
-            host_call()
+            host_call(img, opts, out, flags)
         samples = []
         for _ in range(16):
             t0 = time.perf_counter()
-            host_call()
+            host_call(img, opts, out, flags)
             samples.append(1e3 * (time.perf_counter() - t0))
-        ms512 = float(np.median(samples))
-        result["host_call_512"] = {"value": round(0.262144 / (ms512 / 1e3), 2), "unit": "Mpix/s",
-                                   "ms_per_frame": round(ms512, 3),
-                                   "frame": "512x512 synthetic, error map 5, settings (1, 5, 5), maxiter 4",
-                                   "note": "complete host calls (validation, allocation, copies); "
-                                           "median of 16 after 4 warmups"}
-        log(f"host call 512x512: {ms512:.3f} ms")
-        del d, e, out_h, mask_h
+        ms = float(np.median(samples))
+        exact = bool((out.view(np.uint64) == ref.astype(np.float64).view(np.uint64)).all()
+                     and (flags.astype(bool) == crmask).all())
+        result["host_call_512"] = {"value": round(0.262144 / (ms / 1e3), 2), "unit": "Mpix/s",
+                                   "ms_per_frame": round(ms, 3), "first_call_ms": round(first, 3),
+                                   "frame": "the reference's saved 512x512 scene, error map 5, settings "
+                                            "(1, 5, 5), maxiter 4",
+                                   "matches_saved_reference": exact,
+                                   "note": "complete host calls (validation, allocation, copies); median of 16 "
+                                           "after 4 warmups"}
+        log(f"host call, saved 512x512 scene: {ms:.3f} ms, matches the saved reference: {exact}")
+        return result
 
-    if not args.no_backward:
-        log("forward+backward, 8 x 1024^2 noise-model frames")
+    # ----------------------------------------------------------- fwd_bwd
+    def fwd_bwd(self):
+        torch = self.torch
         from cusmic_amd import autograd
 
-        adj = scenes.adjoint_batch(dev)
+        log("forward+backward, 8 x 1024^2 noise-model frames")
+        adj = self.scenes.adjoint_batch(self.dev)
         data, w = adj["data"], adj["weights"]
-        mp4 = data.numel() / 1e6
+        mp = data.numel() / 1e6
 
         def fwd():
             x = data.clone().requires_grad_(True)
@@ -293,7 +324,7 @@ def main():
         for _ in range(2):
             fwd_bwd()
         torch.cuda.synchronize()
-        reps = max(2, args.steps // 2)
+        reps = max(4, self.args.steps)
         t0 = time.perf_counter()
         for _ in range(reps):
             fwd()
@@ -304,43 +335,40 @@ def main():
             fwd_bwd()
         torch.cuda.synchronize()
         fb_ms = 1e3 * (time.perf_counter() - t0) / reps
-        result["fwd_bwd"] = {"value": round(mp4 / (fb_ms / 1e3), 2), "unit": "Mpix/s",
-                             "forward_mpix_s": round(mp4 / (f_ms / 1e3), 2),
-                             "ms_forward": round(f_ms, 3), "ms_forward_backward": round(fb_ms, 3),
-                             "ratio": round(fb_ms / f_ms, 3), "frames": "8 x 1024x1024"}
         log(f"fwd {f_ms:.3f} ms, fwd+bwd {fb_ms:.3f} ms")
+        return {"value": round(mp / (fb_ms / 1e3), 2), "unit": "Mpix/s",
+                "forward_mpix_s": round(mp / (f_ms / 1e3), 2), "ms_forward": round(f_ms, 3),
+                "ms_forward_backward": round(fb_ms, 3), "ratio": round(fb_ms / f_ms, 3),
+                "frames": "8 x 1024x1024"}
 
-    if not args.no_cpu:
+    # --------------------------------------------------------------- cpu
+    def cpu(self):
+        np = self.np
         import oracle
 
+        result = {}
         cores, model = cpu_info()
-        # the first 1 Mpx frame of the batch, and where its pixels sit in the
-        # ragged call's output
+        batch = self.batch
         offs = np.cumsum([0] + [b["data"].numel() for b in batch])
         pick = next((k for k, b in enumerate(batch) if b["data"].numel() == 1 << 20), 0)
-        sample = [batch[pick]]
-        frames = [dict(data=b["data"].cpu().numpy(), error=b["error"].cpu().numpy()) for b in sample]
-        spx = sum(f["data"].size for f in frames) / 1e6
+        frames = [dict(data=batch[pick]["data"].cpu().numpy(), error=batch[pick]["error"].cpu().numpy())]
+        spx = frames[0]["data"].size / 1e6
         log(f"cpu baseline: oracle, 1 thread, {spx:.2f} Mpx")
         t0 = time.perf_counter()
         o_clean, o_mask, _, _ = oracle.clean_frames(frames, threads=1)
         c_s = time.perf_counter() - t0
         # outside the timed regions: the ragged call's frame equals the oracle's
         lo, hi = int(offs[pick]), int(offs[pick + 1])
-        g_clean = clean[lo:hi].cpu().numpy().reshape(o_clean[0].shape)
-        g_mask = mask[lo:hi].cpu().numpy().reshape(o_mask[0].shape).astype(bool)
-        oracle_equal = bool((g_clean.view(np.uint64) == o_clean[0].view(np.uint64)).all()
-                            and (g_mask == o_mask[0]).all())
+        g_clean = self.clean[lo:hi].cpu().numpy().reshape(o_clean[0].shape)
+        g_mask = self.mask[lo:hi].cpu().numpy().reshape(o_mask[0].shape).astype(bool)
+        equal = bool((g_clean.view(np.uint64) == o_clean[0].view(np.uint64)).all() and (g_mask == o_mask[0]).all())
         shape = "x".join(map(str, frames[0]["data"].shape))
-        result["cpu_baseline"] = {"value": round(spx / c_s, 4), "unit": "Mpix/s", "cores": 1,
-                                  "kind": "port", "cpu": model, "host_cores": cores,
-                                  "sample": f"frame {pick} ({shape}) of the same ragged batch "
-                                            f"(seed 3), error map, maxiter 4; {c_s:.1f} s",
-                                  "gpu_output_equals_oracle": oracle_equal}
-        log(f"cpu: oracle equals the ragged call's frame {pick}: {oracle_equal}")
-        log(f"cpu: {spx / c_s:.3f} Mpix/s")
-        # the same oracle on every host core (OpenMP over rows), on up to four
-        # 1 Mpx frames of the batch
+        result["cpu_baseline"] = {"value": round(spx / c_s, 4), "unit": "Mpix/s", "cores": 1, "kind": "port",
+                                  "cpu": model, "host_cores": cores,
+                                  "sample": f"frame {pick} ({shape}) of the same ragged batch (seed 3), "
+                                            f"error map, maxiter 4; {c_s:.1f} s",
+                                  "gpu_output_equals_oracle": equal}
+        log(f"cpu: {spx / c_s:.3f} Mpix/s; the oracle equals the ragged call's frame {pick}: {equal}")
         wide = [b for b in batch if b["data"].numel() == 1 << 20][:4] or batch[:1]
         wframes = [dict(data=b["data"].cpu().numpy(), error=b["error"].cpu().numpy()) for b in wide]
         wpx = sum(f["data"].size for f in wframes) / 1e6
@@ -349,12 +377,55 @@ def main():
         oracle.clean_frames(wframes, threads=cores)
         a_s = time.perf_counter() - t0
         result["cpu_baseline_all_cores"] = {
-            "value": round(wpx / a_s, 4), "unit": "Mpix/s", "cores": cores, "kind": "port",
-            "cpu": model,
-            "sample": f"{len(wframes)} frame(s) of "
-                      f"{'x'.join(map(str, wframes[0]['data'].shape))} from the same ragged batch, "
-                      f"error map, maxiter 4, OpenMP over rows; {a_s:.1f} s"}
+            "value": round(wpx / a_s, 4), "unit": "Mpix/s", "cores": cores, "kind": "port", "cpu": model,
+            "sample": f"{len(wframes)} frame(s) of {'x'.join(map(str, wframes[0]['data'].shape))} from the same "
+                      f"ragged batch, error map, maxiter 4, OpenMP over rows; {a_s:.1f} s"}
         log(f"cpu, {cores} threads: {wpx / a_s:.3f} Mpix/s")
+        return result
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--frames", type=int, default=64)
+    ap.add_argument("--no-per-frame", action="store_true")
+    ap.add_argument("--no-detector", action="store_true")
+    ap.add_argument("--no-backward", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-host", action="store_true")
+    ap.add_argument("--only", choices=("detector", "fwd-bwd"),
+                    help="run this leg alone (for profiling) and print its record")
+    ap.add_argument("--calibration-copy", action="store_true",
+                    help="start with a 1 GiB device-to-device copy, whose hardware counter "
+                         "readings scale FETCH_SIZE/WRITE_SIZE to bytes")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only; on several GPUs the frames of a batch would be "
+              "sharded by pixel count, with no exchange between GPUs", file=sys.stderr)
+        sys.exit(2)
+    b = Bench(args)
+    if args.calibration_copy:
+        a = b.torch.ones(1 << 27, dtype=b.torch.float64, device=b.dev)
+        b.torch.empty_like(a).copy_(a)
+        b.torch.cuda.synchronize()
+        del a
+    if args.only:
+        leg = b.detector() if args.only == "detector" else b.fwd_bwd()
+        print(json.dumps({"leg": args.only, **leg}), flush=True)
+        return
+    result = b.ragged()
+    if not args.no_per_frame:
+        result["per_frame"] = b.per_frame()
+    if not args.no_detector:
+        result["detector"] = b.detector()
+    if not args.no_host:
+        result.update(b.host())
+    if not args.no_backward:
+        result["fwd_bwd"] = b.fwd_bwd()
+    if not args.no_cpu:
+        result.update(b.cpu())
     print(json.dumps(result), flush=True)
 
 
