@@ -313,19 +313,22 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 		new_pixels[k] = 0;
 	/* the first pass's kernel, and how many of its workgroups fit at once */
 	bool model = false; /* some frame uses the noise model instead of an error map */
-	bool bg_any = !std::isnan(o.background); /* some frame has a background */
+	bool bg_map = false; /* some frame has a background map */
 	for (size_t f = 0; f < nf; ++f) {
 		model |= ims[f].error == nullptr;
-		bg_any |= ims[f].background != nullptr;
+		bg_map |= ims[f].background != nullptr;
 	}
+	const bool bg_any = bg_map || !std::isnan(o.background); /* some frame has a background */
 	using kernel_t = void (*)(Params, int);
 	kernel_t sig;
 	if (model)
 		sig = laplacian ? (bg_any ? significance_model<false, true> : significance_model<false, false>)
 				: (bg_any ? significance_model<true, true> : significance_model<true, false>);
 	else
-		sig = laplacian ? (bg_any ? significance_maps<false, true> : significance_maps<false, false>)
-				: (bg_any ? significance_maps<true, true> : significance_maps<true, false>);
+		sig = laplacian ? (bg_map ? significance_maps<false, true, true>
+				       : bg_any ? significance_maps<false, true, false> : significance_maps<false, false, false>)
+				: (bg_map ? significance_maps<true, true, true>
+				       : bg_any ? significance_maps<true, true, false> : significance_maps<true, false, false>);
 	const int cus = device_cus();
 	int sig_per_cu = 0;
 	hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&sig_per_cu, sig, BLOCK, 0));

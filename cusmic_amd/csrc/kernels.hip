@@ -278,62 +278,61 @@ frame_from(const Frame *frames, int nf, int f, long long g)
 }
 
 /* Epilogue (src/image.cuh:50-70): out = finite D ? C - bg : D (payload bits
- * kept), mask = flagged bit. Eight batch pixels per lane and step, so that
- * the byte planes move in 64-bit accesses: the mask bytes come from one load
- * of the flags plane and leave in one store (bytewise where the caller's mask
- * is not 8-byte aligned). Holes and background maps take a per-pixel path. */
+ * kept), mask = flagged bit. A workgroup step covers 8 BLOCK batch pixels.
+ * The mask takes eight pixels per lane, so that the byte planes move in 64-bit
+ * accesses (bytewise where the caller's mask is not 8-byte aligned). The clean
+ * image is then walked with consecutive pixels in consecutive lanes, and only
+ * in steps that hold a hole or lose a background. */
 __global__ __launch_bounds__(BLOCK) void
 epilogue(Params p, long long total, int bg_maps)
 {
 	constexpr unsigned long long LOW = 0x0101010101010101ull;
+	constexpr int SPAN = 8 * BLOCK;
 	static_assert(HOLE == 1 && CRAY == 4, "bit positions used below");
 	const bool vec_mask = !(reinterpret_cast<uintptr_t>(p.mask) & 7);
 	const bool bg_scalar = !isnan(p.bg);
-	const long long groups = (total + 7) / 8;
-	for (long long G = blockIdx.x * (long long)BLOCK + threadIdx.x; G < groups;
-	     G += (long long)gridDim.x * BLOCK) {
-		const long long g0 = G * 8;
-		const bool full = g0 + 8 <= total;
+	for (long long b0 = (long long)blockIdx.x * SPAN; b0 < total; b0 += (long long)gridDim.x * SPAN) {
+		const long long g0 = b0 + 8 * threadIdx.x;
 		unsigned long long fl = 0;
-		if (full) {
+		if (g0 + 8 <= total) {
 			fl = *reinterpret_cast<const unsigned long long *>(p.flags + g0);
 		} else {
 			for (int k = 0; g0 + k < total; ++k)
 				fl |= (unsigned long long)p.flags[g0 + k] << (8 * k);
 		}
 		const unsigned long long m = (fl >> 2) & LOW;
-		if (full && vec_mask) {
+		if (g0 + 8 <= total && vec_mask) {
 			*reinterpret_cast<unsigned long long *>(p.mask + g0) = m;
 		} else {
 			for (int k = 0; k < 8 && g0 + k < total; ++k)
 				p.mask[g0 + k] = uint8_t(m >> (8 * k));
 		}
-		if ((fl & LOW) || bg_maps) {
-			int f = 0;
-			{
-				int lo = 0, hi = p.nf - 1;
-				while (lo < hi) {
-					int mid = (lo + hi + 1) >> 1;
-					if (p.frames[mid].off <= g0)
-						lo = mid;
-					else
-						hi = mid - 1;
-				}
-				f = lo;
+		if (!__syncthreads_or((fl & LOW) != 0 || bg_maps || bg_scalar))
+			continue;
+		int f = 0;
+		{
+			const long long g = min(b0 + threadIdx.x, total - 1);
+			int lo = 0, hi = p.nf - 1;
+			while (lo < hi) {
+				const int mid = (lo + hi + 1) >> 1;
+				if (p.frames[mid].off <= g)
+					lo = mid;
+				else
+					hi = mid - 1;
 			}
-			for (int k = 0; k < 8 && g0 + k < total; ++k) {
-				const long long g = g0 + k;
-				f = frame_from(p.frames, p.nf, f, g);
-				const Frame &fr = p.frames[f];
-				const long long i = g - fr.off;
-				if ((fl >> (8 * k)) & HOLE)
-					p.clean[g] = fr.data[i];
-				else if (fr.background || bg_scalar)
-					p.clean[g] = p.clean[g] + -1 * (fr.background ? fr.background[i] : p.bg);
-			}
-		} else if (bg_scalar) {
-			for (int k = 0; k < 8 && g0 + k < total; ++k)
-				p.clean[g0 + k] = p.clean[g0 + k] + -1 * p.bg;
+			f = lo;
+		}
+		for (int k = 0; k < 8; ++k) {
+			const long long g = b0 + k * BLOCK + threadIdx.x;
+			if (g >= total)
+				break;
+			f = frame_from(p.frames, p.nf, f, g);
+			const Frame &fr = p.frames[f];
+			const long long i = g - fr.off;
+			if (p.flags[g] & HOLE)
+				p.clean[g] = fr.data[i];
+			else if (fr.background || bg_scalar)
+				p.clean[g] = p.clean[g] + -1 * (fr.background ? fr.background[i] : p.bg);
 		}
 	}
 }
@@ -530,8 +529,9 @@ expand_dirty(Params p, int iter)
 
 /* Value of a replacement from the two middle donors (src/replace.cuh:141-150):
  * odd count -> 0.0 + lo; even -> overflow-safe midpoint. When both middles are
- * zeros, the key order (-0 below +0) decides the sign: -0 iff more than n/2
- * donors have a key at most key(-0) (neg0 counts them). */
+ * zeros, the key order (-0 below +0) that the reference uses for windows above
+ * 32 pixels decides the sign: -0 iff more than n/2 donors have a key at most
+ * key(-0) (neg0 counts them); smaller windows take network_zero. */
 __device__ __forceinline__ double
 replacement(int n, double lo, double hi, int neg0)
 {
@@ -540,6 +540,52 @@ replacement(int n, double lo, double hi, int neg0)
 	if (lo == 0.0 && hi == 0.0)
 		return neg0 > (n >> 1) ? -0.0 : 0.0;
 	return midpoint(lo, hi);
+}
+
+/* The sign of an even-count replacement whose two middle donors are zeros, in
+ * a window of at most 32 pixels. The reference sorts such windows by value on
+ * a 32-lane bitonic network (src/replace.cuh:68-83,132-145): lane pos holds the
+ * pixel at raster position pos of the window if it is a donor, else +inf, and
+ * a compare-exchange moves values only when they differ, so -0 and +0 keep
+ * their lanes against each other. Whether an exchange that involves a zero
+ * happens depends only on the classes (negative, zero, positive) of the two
+ * lanes, and exchanges within a class leave the class layout as it was, so the
+ * network runs on three lane masks: negatives, zeros, and the zeros that are
+ * -0. The midpoint of two zeros is -0 iff both are -0. */
+__device__ double
+network_zero(const uint8_t *flags, const double *clean, int w, int x0, int x1, int y0, int y1, int n)
+{
+	const int ww = x1 - x0 + 1, area = ww * (y1 - y0 + 1);
+	uint32_t neg = 0, zero = 0, nzero = 0;
+	for (int k = 0; k < area; ++k) {
+		double v;
+		if (is_donor(flags, clean, (y0 + k / ww) * w + x0 + k % ww, v)) {
+			neg |= uint32_t(v < 0) << k;
+			zero |= uint32_t(v == 0.0) << k;
+			nzero |= uint32_t(v == 0.0 && signbit(v)) << k;
+		}
+	}
+	/* LANES[t]: lanes whose bit t is clear */
+	constexpr uint32_t LANES[5] = {0x55555555u, 0x33333333u, 0x0f0f0f0fu, 0x00ff00ffu, 0x0000ffffu};
+	for (int s = 1; s <= 5; ++s) {     /* span 2^s */
+		const uint32_t keeps_min = s < 5 ? LANES[s] : ~0u;
+		for (int t = s - 1; t >= 0; --t) { /* step 2^t: lane l pairs with l + step */
+			const int step = 1 << t;
+			const uint32_t pos = ~(neg | zero);
+			const uint32_t nu = neg >> step, zu = zero >> step, pu = pos >> step;
+			const uint32_t l_above = (zero & nu) | (pos & (nu | zu));
+			const uint32_t u_above = (zu & neg) | (pu & (neg | zero));
+			const uint32_t swap = LANES[t] & ((keeps_min & l_above) | (~keeps_min & u_above));
+			uint32_t d = (neg ^ nu) & swap;
+			neg ^= d | (d << step);
+			d = (zero ^ zu) & swap;
+			zero ^= d | (d << step);
+			d = (nzero ^ (nzero >> step)) & swap;
+			nzero ^= d | (d << step);
+		}
+	}
+	const int a = (n - 1) >> 1, b = n >> 1;
+	return (nzero >> a) & (nzero >> b) & 1u ? -0.0 : 0.0;
 }
 
 /* Store the replacement of pixel i; record the donors when asked (ties in
@@ -551,6 +597,8 @@ store_replacement(const Params &p, const Frame &fr, int i, int32_t *donors, int 
 	const double *clean = p.clean + fr.off;
 	const uint8_t *flags = p.flags + fr.off;
 	double out = replacement(n, lo, hi, neg0);
+	if (!(n & 1) && lo == 0.0 && hi == 0.0 && (x1 - x0 + 1) * (y1 - y0 + 1) <= 32)
+		out = network_zero(flags, clean, fr.w, x0, x1, y0, y1, n);
 	if (donors) {
 		int a = (n - 1) >> 1, b = n >> 1;
 		int ia = rank_index(flags, clean, fr.w, x0, x1, y0, y1, rank_key(lo, a, neg0), a);
@@ -1424,7 +1472,7 @@ model_row_region(SigShared &sh, int &slot, int lane, double t_min, unsigned long
  * the mode is WRAP, under which every stencil commutes with the periodic
  * extension. NR is a constant so that the row loop unrolls around its ballots
  * and the loads of several rows are in flight. */
-template <bool MODEL, bool EXACT, bool BG, int NR>
+template <bool MODEL, bool EXACT, bool BG, bool BGMAP, int NR>
 __device__ __forceinline__ void
 significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0, int yb, int r0,
 	double t_min, unsigned long long cols, int own0, int own1, OwnCounts &acc,
@@ -1450,16 +1498,16 @@ significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0
 		using gdouble = const __attribute__((address_space(1))) double;
 		const long long o = (long long)ya * w + gx, W = w;
 		gdouble *dp = (gdouble *)(fr.data + o), *ep = (gdouble *)(fr.error + o);
-		gdouble *bp = BG && fr.background ? (gdouble *)(fr.background + o) : nullptr;
+		gdouble *bp = BGMAP && fr.background ? (gdouble *)(fr.background + o) : nullptr;
 		/* the left and right samples of a row are the neighbouring lanes'
 		 * centres (a wave shift). Lanes 0 and 63 take theirs from two scalar
 		 * loads per row (the address is the wave's; the input is read-only
 		 * here), which leaves one vector load fewer per row and the registers
 		 * for more rows in flight. This is the error-map kernel without a
-		 * background (SEDGE); elsewhere the edge samples are vector loads:
+		 * background map (SEDGE); elsewhere the edge samples are vector loads:
 		 * lanes 0 and 63 load theirs, the others reload their centre in the
 		 * same instruction, two rows ahead. */
-		constexpr bool SEDGE = !MODEL && !BG;
+		constexpr bool SEDGE = !MODEL && !BGMAP;
 		const int eo = lane == 0 ? -1 : lane == 63 ? 1 : 0;
 		using cdouble = const __attribute__((address_space(4))) double;
 		const unsigned long long lb = uniform_u64((unsigned long long)(fr.data + (long long)ya * w + (X0 - HS - 1)));
@@ -1585,7 +1633,7 @@ significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0
  * after this pass, so its blocks go to the dirty list of the block pass that
  * follows the fill (tile_skip marks it). Elsewhere S' is exact, and the
  * screened pixels carry SCR. */
-template <bool MODEL, bool EXACT, bool BG>
+template <bool MODEL, bool EXACT, bool BG, bool BGMAP>
 __device__ __forceinline__ void
 significance_pass(SigShared &sh, ModelRegion *region, const Params &p, int iter)
 {
@@ -1645,11 +1693,11 @@ significance_pass(SigShared &sh, ModelRegion *region, const Params &p, int iter)
 			}
 			if (s == 0) {
 				const int r0 = wave * (STEP_ROWS / 4);
-				significance_rows<MODEL, EXACT, BG, STEP_ROWS / 4>(sh, p, fr, f, X0, yb, r0, t_min,
+				significance_rows<MODEL, EXACT, BG, BGMAP, STEP_ROWS / 4>(sh, p, fr, f, X0, yb, r0, t_min,
 					cols, Y0, own1, acc, region, rS0);
 			} else {
 				const int r0 = TILE_H * s + 2 * HS + wave * (TILE_H / 4);
-				significance_rows<MODEL, EXACT, BG, TILE_H / 4>(sh, p, fr, f, X0, yb, r0, t_min,
+				significance_rows<MODEL, EXACT, BG, BGMAP, TILE_H / 4>(sh, p, fr, f, X0, yb, r0, t_min,
 					cols, Y0, own1, acc, region, rS0);
 			}
 			const int ty = Y0 + TILE_H * s; /* first frame row of the tile */
@@ -2080,13 +2128,14 @@ significance_blocks(Params p, int iter)
 /* The first pass when every frame has an error map. It is bound by load
  * latency, so the register budget is capped at four waves per SIMD (128
  * VGPRs, no spills), one workgroup more per CU than the compiler would choose.
- * BG: some frame has a background. */
-template <bool EXACT, bool BG>
+ * BG: some frame has a background; BGMAP: some frame has a background map. */
+template <bool EXACT, bool BG, bool BGMAP>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
 significance_maps(Params p, int iter)
 {
+	static_assert(BG || !BGMAP, "a background map is a background");
 	__shared__ SigShared sh;
-	significance_pass<false, EXACT, BG>(sh, nullptr, p, iter);
+	significance_pass<false, EXACT, BG, BGMAP>(sh, nullptr, p, iter);
 }
 
 /* Some frame uses the noise model (5x5 medians of C in the dense part):
@@ -2097,7 +2146,7 @@ significance_model(Params p, int iter)
 {
 	__shared__ SigShared sh;
 	__shared__ ModelRegion region;
-	significance_pass<true, EXACT, BG>(sh, &region, p, iter);
+	significance_pass<true, EXACT, BG, BG>(sh, &region, p, iter);
 }
 
 __device__ __forceinline__ double

@@ -75,6 +75,7 @@ def lib():
         _lib.oracle_border_index.restype = ctypes.c_long
         _lib.oracle_replace.argtypes = [p, p, p, ctypes.c_long, ctypes.c_long, p]
         _lib.oracle_replace.restype = ctypes.c_int
+        _lib.oracle_bitonic32.argtypes = [p]
     return _lib
 
 
@@ -117,6 +118,14 @@ def replace(clean, targets, donors, record=False):
     if status == -1:
         raise ValueError("no finite replacement donors")
     return idx
+
+
+def bitonic32(values):
+    """The reference's 32-lane value network applied to 32 doubles."""
+    v = np.array(values, dtype=np.float64)
+    assert v.shape == (32,)
+    lib().oracle_bitonic32(_ptr(v))
+    return v
 
 
 class OracleError(Exception):

@@ -212,12 +212,40 @@ midpoint(double lo, double hi)
 	return lo / 2 + hi / 2;
 }
 
+/* The reference's replacement sort for windows of at most 32 pixels
+ * (src/replace.cuh:68-83,132-145): a 32-lane bitonic network in which lane pos
+ * holds the pixel at raster position pos of the clipped window (+inf for a
+ * non-donor and past the window). In each compare-exchange the lane that keeps
+ * the minimum takes its partner's value only when that is strictly smaller,
+ * so +0 and -0 stay where they are against each other. */
+void
+oracle_bitonic32(double v[32])
+{
+	for (int span = 2; span <= 32; span *= 2)
+		for (int step = span / 2; step; step /= 2)
+			for (int l = 0; l < 32; ++l) {
+				int u = l | step;
+				if (l & step)
+					continue;
+				double a = v[l], b = v[u];
+				/* l keeps the minimum when its span bit is clear */
+				int swap = !(l & span) ? b < a : a < b;
+				if (swap) {
+					v[l] = b;
+					v[u] = a;
+				}
+			}
+}
+
 /* Replace every target with the median of the donors in the smallest clipped
  * window [x-r, x+r] x [y-r, y+r], r = 2, 3, ..., that holds a donor. Donor
  * values come from the pre-pass snapshot (targets are never donors), odd
  * counts give 0.0 + middle and even counts the midpoint of the two middles.
- * Ties order by value key, then by pixel index (src/replace.cuh:112-153;
- * mod/cusmic/replace.py:28-70). */
+ * Values come from the reference's value network for windows of at most 32
+ * pixels and from key order (-0 below +0) for larger ones; the two differ only
+ * in the sign of an even-count median whose two middle donors are zeros. The
+ * recorded donors order ties by value key, then by pixel index
+ * (src/replace.cuh:112-153; mod/cusmic/replace.py:28-70). */
 int
 oracle_replace(double *clean, const uint8_t *target, const uint8_t *donor, long w, long h,
 	int32_t *donors)
@@ -262,7 +290,19 @@ oracle_replace(double *clean, const uint8_t *target, const uint8_t *donor, long 
 				continue;
 			qsort(list, cnt, sizeof(*list), cmp_donor);
 			struct donor lo = list[(cnt - 1) / 2], hi = list[cnt / 2];
-			next[i] = cnt % 2 ? 0.0 + lo.value : midpoint(lo.value, hi.value);
+			double vlo = lo.value, vhi = hi.value;
+			if (need <= 32) {
+				double v[32];
+				long ww = x1 - x0 + 1;
+				for (int k = 0; k < 32; ++k) {
+					long j = (y0 + k / ww) * w + x0 + k % ww;
+					v[k] = (size_t)k < need && donor[j] && isfinite(clean[j]) ? clean[j] : INFINITY;
+				}
+				oracle_bitonic32(v);
+				vlo = v[(cnt - 1) / 2];
+				vhi = v[cnt / 2];
+			}
+			next[i] = cnt % 2 ? 0.0 + vlo : midpoint(vlo, vhi);
 			if (donors) {
 				donors[2 * i] = (int32_t)lo.index;
 				donors[2 * i + 1] = (int32_t)hi.index;

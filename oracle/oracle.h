@@ -40,6 +40,8 @@ long oracle_border_index(long x, long n, int mode);
  * success or -1 when a target finds no donor anywhere. */
 int oracle_replace(double *clean, const uint8_t *target, const uint8_t *donor, long w, long h,
 	int32_t *donors);
+/* The reference's 32-lane value network for small replacement windows. */
+void oracle_bitonic32(double v[32]);
 
 #ifdef __cplusplus
 }

@@ -536,3 +536,53 @@ def test_background_overflow(cuda):
         assert want[1][0].sum() > 10
     say("background overflow, noise model")
     compare([dict(data=d, mask=m, background=bg)], frame_gain=[2.0], frame_readnoise=[5.0])
+    say("background overflow, scalar background")
+    from cusmic_amd import Cleaner, Image
+
+    # scaled so that a scalar background overflows the DBL_MAX pixels only
+    ds, es = d * 1e290, err * 1e290
+    ds[big] = np.finfo(np.float64).max
+    dev = torch.device("cuda", 0)
+    for mode in ("mirror", "wrap"):
+        outs, masks, _, passes = oracle.clean_frames([dict(data=ds, error=es, mask=m)], background=3e292,
+                                                    border_mode=mode)
+        clean, mask, _, gpasses = Cleaner(border_mode=mode).run(
+            Image(torch.tensor(ds, device=dev), error=torch.tensor(es, device=dev),
+                  mask=torch.tensor(m, device=dev), background=3e292))
+        np.testing.assert_array_equal(np_of(mask), masks[0])
+        assert_same_pixels(np_of(clean), outs[0])
+        assert list(gpasses) == list(passes) and masks[0].sum() > 10
+
+
+def zero_scene(shape, rng):
+    """+0 and -0 everywhere, a few +-1, rays of 40 and runs of up to 7 pixels
+    of 400 along rows (widened windows), holes and masked pixels."""
+    h, w = shape
+    d = rng.choice(np.array([-0.0, 0.0]), shape)
+    d[rng.random(shape) < 0.08] = rng.choice([-1.0, 1.0])
+    d[rng.random(shape) < 0.04] = 40.0
+    for _ in range(max(2, h * w // 200)):
+        y, x = rng.integers(h), rng.integers(w)
+        d[y, x:x + rng.integers(2, 8)] = 400.0
+    d[rng.random(shape) < 0.01] = np.nan
+    return d, rng.random(shape) < 0.03
+
+
+def test_signed_zero_ties(cuda):
+    """Replacement windows whose middle donors are zeros of both signs: the
+    sign is the one the reference's 32-lane value network leaves for windows
+    of at most 32 pixels (also widened windows of narrow frames), the key
+    order for larger ones, as the oracle computes them."""
+    rng = np.random.default_rng(29)
+    negative = positive = 0
+    for shape in ((64, 96), (1, 80), (2, 70), (7, 5), (40, 3)):
+        say("signed zero ties", shape)
+        d, m = zero_scene(shape, rng)
+        for mode in ("mirror", "constant"):
+            got, want = compare([dict(data=d, error=np.ones(shape), mask=m)], border_mode=mode,
+                                contrast=1, record_donors=True)
+            out, flagged = want[0][0], want[1][0] | np.isnan(d)
+            zero = flagged & (out == 0)
+            negative += int((zero & np.signbit(out)).sum())
+            positive += int((zero & ~np.signbit(out)).sum())
+    assert negative > 20 and positive > 20

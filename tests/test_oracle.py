@@ -120,9 +120,30 @@ def test_border_index_matches_numpy_pad():
             np.testing.assert_array_equal(got, padded)
 
 
+def warp_network(values):
+    """The reference's warp sort (src/replace.cuh:68-83) lane by lane: in each
+    step every lane reads its xor partner and keeps the minimum or the maximum,
+    its own value on a tie."""
+    v = np.array(values, dtype=np.float64)
+    lane = np.arange(32)
+    span = 2
+    while span <= 32:
+        step = span // 2
+        while step:
+            peer = v[lane ^ step]
+            lower = ((lane & span) == 0) == ((lane & step) == 0)
+            v = np.where(lower, np.where(peer < v, peer, v), np.where(v < peer, peer, v))
+            step //= 2
+        span *= 2
+    return v
+
+
 def direct_replace(clean, targets, donors):
-    """Straight numpy statement of the replacement (mod/cusmic/replace.py:28-70)
-    with ties ordered by value key then index."""
+    """Straight numpy statement of the replacement (src/replace.cuh:112-150):
+    windows of at most 32 pixels sort on the warp network with the window in
+    raster order over the lanes (+inf for non-donors), larger ones by key
+    (numpy's sort puts -0 and +0 as equals in input order, so the key order
+    -0 below +0 is made explicit)."""
     h, w = clean.shape
     out = clean.copy()
     for y, x in zip(*np.nonzero(targets)):
@@ -132,8 +153,13 @@ def direct_replace(clean, targets, donors):
             ok = donors[y0:y1, x0:x1] & np.isfinite(win)
             vals = win[ok]
             if vals.size:
-                vals = np.sort(vals, kind="stable")
                 n = vals.size
+                if win.size <= 32:
+                    lanes = np.full(32, np.inf)
+                    lanes[:win.size] = np.where(ok, win, np.inf).ravel()
+                    vals = warp_network(lanes)
+                else:
+                    vals = vals[np.lexsort((~np.signbit(vals), vals))]
                 lo, hi = vals[(n - 1) // 2], vals[n // 2]
                 if n % 2:
                     out[y, x] = 0.0 + lo
@@ -158,6 +184,54 @@ def test_replacement_brute_force():
         got = clean.copy()
         oracle.replace(got, targets, donors)
         assert_same_pixels(got, want)
+
+
+def test_bitonic32_is_the_reference_warp_sort():
+    """The oracle's pairwise network equals the lane-by-lane statement, sorts
+    by value, and leaves -0 and +0 where the network puts them."""
+    rng = np.random.default_rng(8)
+    moved = 0
+    for trial in range(300):
+        v = rng.choice([-2.0, -1.0, -0.0, 0.0, 1.0, np.inf], 32)
+        got = oracle.bitonic32(v)
+        assert_same_pixels(got, warp_network(v))
+        np.testing.assert_array_equal(got, np.sort(v))
+        key_order = np.sort(np.signbit(v[v == 0]))[::-1]  # -0 before +0
+        moved += not np.array_equal(np.signbit(got[got == 0]), key_order)
+    assert moved > 100
+    np.testing.assert_array_equal(oracle.bitonic32(rng.permutation(32).astype(float)), np.arange(32.0))
+
+
+def test_replacement_zero_signs():
+    """Replacement windows full of +0 and -0: an even-count median of two zero
+    middles is -0 only when the network leaves -0 in both middle lanes, which
+    the key order (-0 below +0) does not always give; windows above 32 pixels
+    do use the key order."""
+    rng = np.random.default_rng(9)
+    differs = 0
+    for trial in range(300):
+        h, w = rng.integers(1, 9, size=2)
+        clean = rng.choice([-1.0, -0.0, 0.0, 1.0], (h, w), p=[0.2, 0.3, 0.3, 0.2])
+        targets = rng.random((h, w)) < 0.3
+        donors = ~targets & (rng.random((h, w)) < 0.85)
+        if not donors.any() or not targets.any():
+            continue
+        want = direct_replace(clean, targets, donors)
+        got = clean.copy()
+        oracle.replace(got, targets, donors)
+        assert_same_pixels(got, want)
+        by_key = clean.copy()
+        for y, x in zip(*np.nonzero(targets)):
+            for r in range(2, max(h, w) + 1):
+                y0, y1, x0, x1 = max(0, y - r), min(h, y + r + 1), max(0, x - r), min(w, x + r + 1)
+                vals = clean[y0:y1, x0:x1][donors[y0:y1, x0:x1]]
+                if vals.size:
+                    vals = vals[np.lexsort((~np.signbit(vals), vals))]
+                    n = vals.size
+                    by_key[y, x] = 0.0 + vals[n // 2] if n % 2 else (vals[(n - 1) // 2] + vals[n // 2]) / 2
+                    break
+        differs += int((bits(by_key) != bits(got)).sum())
+    assert differs > 20
 
 
 def test_replacement_kats():
