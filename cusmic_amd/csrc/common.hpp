@@ -14,9 +14,10 @@
 
 namespace cusmic {
 
-/* Detection tile: 60 x 32 outputs, so that the significance region with the
- * 2-pixel halo of the 5x5 median is exactly one wave (64 lanes) wide. */
-constexpr int TILE_W = 60;
+/* Detection tile: 58 x 32 outputs, so that the significance region with the
+ * 2-pixel halo of the 5x5 median and the 1-pixel ring of the Laplacian is
+ * exactly one wave (64 lanes) wide. */
+constexpr int TILE_W = 58;
 constexpr int TILE_H = 32;
 constexpr int BLOCK = 256;
 

@@ -1097,7 +1097,7 @@ redo_scan(Params p, int iter)
  */
 
 constexpr int HS = 2;                      /* halo of S: the 5x5 median */
-constexpr int SW = TILE_W + 2 * HS;        /* 64: one wave */
+constexpr int SW = TILE_W + 2 * HS + 2;    /* 64: one wave of C columns */
 constexpr int STEP_ROWS = TILE_H + 2 * HS; /* rows of S the medians of one tile read */
 constexpr int RING = 40;                   /* rows of S kept in LDS */
 static_assert(SW == 64, "the significance region is one wave wide");
@@ -1382,7 +1382,7 @@ own_row(const Params &p, const Frame &fr, bool own, long long i, double c, bool 
 
 /* The samples of C that one step of the noise-model first pass reads, in LDS:
  * region row k is run row rS0 - HS + k (rS0: the step's first S row), region
- * column c is frame column X0 - 2 HS + c, both under the median's border
+ * column c is frame column X0 - 2 HS - 1 + c, both under the median's border
  * mode (CONSTANT: 0). The first step has STEP_ROWS S rows and reads the most:
  * its last wave's last four-row strip ends three rows below them. */
 constexpr int CREG_COLS = SW + 2 * HS;                /* 68 */
@@ -1404,9 +1404,9 @@ load_region(ModelRegion &R, const Params &p, const Frame &fr, int X0, int yc0, i
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	const int mode = p.mode, w = fr.w, h = fr.h;
 	const bool has_bg = BG && (fr.background || !isnan(p.bg));
-	const int bx0 = border_index(X0 - 2 * HS + lane, w, mode);
+	const int bx0 = border_index(X0 - 2 * HS - 1 + lane, w, mode);
 	const bool second = lane < CREG_COLS - 64;
-	const int bx1 = second ? border_index(X0 - 2 * HS + 64 + lane, w, mode) : -1;
+	const int bx1 = second ? border_index(X0 - 2 * HS - 1 + 64 + lane, w, mode) : -1;
 	for (int k = wave; k < nrows; k += BLOCK / 64) {
 		const int by = border_index(yc0 + k, h, mode);
 		bool h0 = false, h1 = false;
@@ -1454,24 +1454,30 @@ model_row_region(SigShared &sh, int &slot, int lane, double t_min, unsigned long
 	const double rn = fr.readnoise ? fr.readnoise[mi] : fr.rn;
 	const double nz = noise_model(med, gg, rn);
 	own_row<BG>(p, fr, own, (long long)gy * w + gx, cc, hc, true, acc);
+	/* lanes 0 and 63 carry C only: their S is never read */
+	const bool s_lane = lane >= 1 && lane <= SW - 2;
 	emit_row(sh, slot, lane, t_min, cols,
 		laplacian_q<EXACT, !BG>(cc * 0.25, u * 0.25, d * 0.25, l * 0.25, r * 0.25, w == 1) / (2 * nz),
-		(in || mode == CUSMIC_WRAP) && invalid_noise(nz), hm);
+		s_lane && (in || mode == CUSMIC_WRAP) && invalid_noise(nz), hm);
 }
 
 /* One wave: S of run rows [r0, r0 + NR) into the ring (run row r is frame row
- * yb + r, lane l is column X0 - HS + l), with each row's screened columns
+ * yb + r, lane l is column X0 - HS - 1 + l), with each row's screened columns
  * (among cols), whether it holds an invalid noise value and whether a sample
- * of C it read is a hole. C is made from the input here and stored at once
- * for the run's own pixels: the rows [own0, own1) and the columns of cols
- * (own_row). Storing inside the row loop keeps no C in registers, which
- * leaves them to loads five rows ahead. Without a background no sample of C is infinite,
- * so the Laplacian clips with one max instruction; with one the reference's
- * NaN-keeping clip is used. Slots outside the frame are computed from
- * in-bounds samples; the caller overwrites the ones the medians read, unless
- * the mode is WRAP, under which every stencil commutes with the periodic
- * extension. NR is a constant so that the row loop unrolls around its ballots
- * and the loads of several rows are in flight. */
+ * of C it read is a hole. The S that the medians read are those of lanes
+ * 1..62 (columns X0 - HS .. X0 + TILE_W + HS - 1); lanes 0 and 63 hold the
+ * C samples left and right of them, so every Laplacian neighbour is a
+ * neighbouring lane's centre, and their own S is never read. C is made from
+ * the input here and stored at once for the run's own pixels: the rows
+ * [own0, own1) and the columns of cols (own_row). Storing inside the row loop
+ * keeps no C in registers, which leaves them to loads five rows ahead. Without
+ * a background no sample of C is infinite, so the Laplacian clips with one max
+ * instruction; with one the reference's NaN-keeping clip is used. Slots
+ * outside the frame are computed from in-bounds samples; the caller overwrites
+ * the ones the medians read, unless the mode is WRAP, under which every
+ * stencil commutes with the periodic extension. NR is a constant so that the
+ * row loop unrolls around its ballots and the loads of several rows are in
+ * flight. */
 template <bool MODEL, bool EXACT, bool BG, bool BGMAP, int NR>
 __device__ __forceinline__ void
 significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0, int yb, int r0,
@@ -1488,10 +1494,12 @@ significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0
 	const bool wrap = mode == CUSMIC_WRAP, narrow = w == 1;
 	const bool has_bg = BG && (fr.background || !isnan(p.bg));
 	const bool own_lane = (cols >> lane) & 1;
-	const int gx = X0 - HS + lane, ya = yb + r0;
+	/* lanes whose S the medians may read */
+	const bool s_lane = lane >= 1 && lane <= SW - 2;
+	const int gx = X0 - HS - 1 + lane, ya = yb + r0;
 	int slot = r0 % RING;
 	/* the rows and their one-pixel ring lie inside the frame */
-	const bool inner = X0 - HS - 1 >= 0 && X0 - HS + SW < w && ya - 1 >= 0 && ya + nr < h;
+	const bool inner = X0 - HS - 1 >= 0 && X0 - HS - 1 + SW <= w && ya - 1 >= 0 && ya + nr < h;
 	if ((!MODEL || fr.error) && inner) {
 		/* global-space pointers: flat loads would also count against the
 		 * LDS wait counter */
@@ -1499,39 +1507,18 @@ significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0
 		const long long o = (long long)ya * w + gx, W = w;
 		gdouble *dp = (gdouble *)(fr.data + o), *ep = (gdouble *)(fr.error + o);
 		gdouble *bp = BGMAP && fr.background ? (gdouble *)(fr.background + o) : nullptr;
-		/* the left and right samples of a row are the neighbouring lanes'
-		 * centres (a wave shift). Lanes 0 and 63 take theirs from two scalar
-		 * loads per row (the address is the wave's; the input is read-only
-		 * here), which leaves one vector load fewer per row and the registers
-		 * for more rows in flight. This is the error-map kernel without a
-		 * background map (SEDGE); elsewhere the edge samples are vector loads:
-		 * lanes 0 and 63 load theirs, the others reload their centre in the
-		 * same instruction, two rows ahead. */
-		constexpr bool SEDGE = !MODEL && !BGMAP;
-		const int eo = lane == 0 ? -1 : lane == 63 ? 1 : 0;
-		using cdouble = const __attribute__((address_space(4))) double;
-		const unsigned long long lb = uniform_u64((unsigned long long)(fr.data + (long long)ya * w + (X0 - HS - 1)));
-		cdouble *dl = (cdouble *)lb, *dr = (cdouble *)(lb + 8ull * (SW + 1));
-		/* the samples of row r: the pixel below, the edge samples and the
-		 * noise (and their background), fetched PF rows ahead */
-		constexpr int PF = SEDGE ? 5 : 2;
-		double dn[NR], de[NR], el[NR], er[NR], nz[NR], bn[NR], be[NR];
+		/* the samples of row r: the pixel below and the noise (and the
+		 * background below), fetched PF rows ahead */
+		constexpr int PF = BGMAP ? 4 : 5;
+		double dn[NR], nz[NR], bn[NR];
 #define FETCH_ROW(k)                                                                       \
 	do {                                                                               \
 		const long long ro = (long long)(k) * W;                                   \
 		dn[k] = dp[ro + W];                                                        \
-		if (!SEDGE) {                                                              \
-			de[k] = dp[ro + eo];                                               \
-		} else {                                                                   \
-			el[k] = dl[ro];                                                    \
-			er[k] = dr[ro];                                                    \
-		}                                                                          \
 		nz[k] = ep[ro];                                                            \
-		bn[k] = be[k] = p.bg;                                                      \
-		if (BG && bp) {                                                            \
+		bn[k] = p.bg;                                                              \
+		if (BG && bp)                                                              \
 			bn[k] = bp[ro + W];                                                \
-			be[k] = bp[ro + eo];                                               \
-		}                                                                          \
 	} while (0)
 		bool hu = false, hc = false;
 		double qu = input_c<BG>(dp[-W], bp ? bp[-W] : p.bg, has_bg, hu) * 0.25;
@@ -1546,13 +1533,10 @@ significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0
 		for (int r = 0; r < NR; ++r) {
 			if (r + PF < NR)
 				FETCH_ROW(r + PF);
-			bool hd = false, hs = false;
+			bool hd = false;
 			const double cd = input_c<BG>(dn[r], bn[r], has_bg, hd);
-			const double qe = input_c<BG>(SEDGE ? lane == 0 ? el[r] : er[r] : de[r], be[r], has_bg, hs) * 0.25;
-			/* the other lanes' hole flags come from their own samples */
-			if (SEDGE)
-				hs = hs && (lane == 0 || lane == 63);
-			const double ql = wave_shift_right(qc, qe), qr = wave_shift_left(qc, qe);
+			/* lanes 0 and 63 keep their own centre: their S is not read */
+			const double ql = wave_shift_right(qc, qc), qr = wave_shift_left(qc, qc);
 			const double qd = cd * 0.25;
 			const int gy = ya + r;
 			/* non-temporal: C is read again only by later kernels, so it
@@ -1564,7 +1548,7 @@ significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0
 			own_row<BG>(p, fr, own, o + r * W, cc, hc, false, acc);
 			emit_row(sh, slot, lane, t_min, cols,
 				laplacian_q<EXACT, TAME>(qc, qu, qd, ql, qr, false) / (2 * nz[r]),
-				invalid_noise(nz[r]), hu || hc || hd || hs);
+				s_lane && invalid_noise(nz[r]), hc || (s_lane && (hu || hd)));
 			qu = qc;
 			qc = qd;
 			cc = cd;
@@ -1589,7 +1573,7 @@ significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0
 				acc);
 			emit_row(sh, slot, lane, t_min, cols,
 				laplacian_q<EXACT, TAME>(cc * 0.25, qu, cd * 0.25, ql, qr, narrow) / (2 * nz),
-				(in || wrap) && invalid_noise(nz), hu || hc || hd || hs);
+				s_lane && (in || wrap) && invalid_noise(nz), hc || (s_lane && (hu || hd || hs)));
 			qu = cc * 0.25;
 			cc = cd;
 			hu = hc;
@@ -1671,7 +1655,7 @@ significance_pass(SigShared &sh, ModelRegion *region, const Params &p, int iter)
 		const int K = min(p.run_tiles, fr.tiles_y - ty0);
 		const int w = fr.w, h = fr.h, X0 = tx * TILE_W, Y0 = ty0 * TILE_H, yb = Y0 - HS;
 		const int ncols = min(TILE_W, w - X0), own1 = min(Y0 + TILE_H * K, h);
-		const unsigned long long cols = ((1ull << ncols) - 1) << HS;
+		const unsigned long long cols = ((1ull << ncols) - 1) << (HS + 1);
 		const bool has_bg = BG && (fr.background || !isnan(p.bg));
 		OwnCounts acc{0, 0, 0, 0, 0, false};
 
@@ -1714,12 +1698,12 @@ significance_pass(SigShared &sh, ModelRegion *region, const Params &p, int iter)
 			const bool edge = X0 - HS < 0 || X0 + TILE_W + HS > w || y_lo < 0 || y_lo + STEP_ROWS > h;
 			if (edge && !wrap) {
 				for (int k = tid; k < SW * STEP_ROWS; k += BLOCK) {
-					int sx = k & (SW - 1), sr = k / SW, x = X0 - HS + sx, y = y_lo + sr;
+					int sx = k & (SW - 1), sr = k / SW, x = X0 - HS - 1 + sx, y = y_lo + sr;
 					if ((x >= 0 && x < w && y >= 0 && y < h) || x < -2 || x > w + 1 || y < -2 || y > h + 1)
 						continue;
 					double v = 0.0;
 					if (mode != CUSMIC_CONSTANT)
-						v = sh.s[(border_index(y, h, mode) - yb) % RING][border_index(x, w, mode) - (X0 - HS)];
+						v = sh.s[(border_index(y, h, mode) - yb) % RING][border_index(x, w, mode) - (X0 - HS - 1)];
 					sh.s[(r_lo + sr) % RING][sx] = v;
 				}
 				lds_barrier();
@@ -1769,16 +1753,27 @@ significance_pass(SigShared &sh, ModelRegion *region, const Params &p, int iter)
 			 * holes are flagged above). A screened pixel's flags are written
 			 * once, with SCR, by the medians below. */
 			if (fr.mask) {
-				for (int k = tid; k < TILE_W * TILE_H; k += BLOCK) {
-					const int row = k / TILE_W, col = k % TILE_W;
-					if (row >= nrows || col >= ncols)
+				/* every mask byte of this thread is loaded at once; the data
+				 * only at masked pixels */
+				constexpr int PER = (TILE_W * TILE_H + BLOCK - 1) / BLOCK;
+				uint8_t mv[PER];
+#pragma unroll
+				for (int j = 0; j < PER; ++j) {
+					const int k = tid + j * BLOCK, row = k / TILE_W, col = k % TILE_W;
+					const bool in = k < TILE_W * TILE_H && row < nrows && col < ncols;
+					mv[j] = in ? fr.mask[(ty + row) * w + X0 + col] : 0;
+				}
+#pragma unroll
+				for (int j = 0; j < PER; ++j) {
+					if (!mv[j])
 						continue;
+					const int k = tid + j * BLOCK, row = k / TILE_W, col = k % TILE_W;
 					const int i = (ty + row) * w + X0 + col;
 					const double d = fr.data[i];
-					if (!fr.mask[i] || !isfinite(d))
+					if (!isfinite(d))
 						continue;
 					const bool screened = !skip && (unsafe ||
-						((sh.screen[(r_lo + HS + row) % RING] >> (col + HS)) & 1));
+						((sh.screen[(r_lo + HS + row) % RING] >> (col + HS + 1)) & 1));
 					if (!screened)
 						p.flags[fr.off + i] = uint8_t(EXCL);
 					++acc.masked;
@@ -1803,7 +1798,7 @@ significance_pass(SigShared &sh, ModelRegion *region, const Params &p, int iter)
 				int i = 0;
 				if (idx < na) {
 					int pos = sh.alist[idx], row = pos >> 6, c = pos & 63;
-					i = (ty + row) * w + X0 - HS + c;
+					i = (ty + row) * w + X0 - HS - 1 + c;
 					int rc = r_lo + HS + row;
 					double v[25];
 #pragma unroll
@@ -2457,7 +2452,7 @@ donor_weight_term(const int32_t *donors, const double *grad_clean, long long pg,
  * window's radius.
  *
  * Lane l is column x0 - 2 + l, so a wave's ballot over a row is the row's
- * flagged pixels around the tile's 60 columns (lanes 2..61). Each wave walks
+ * flagged pixels around the tile's TILE_W columns (lanes 2..TILE_W + 1). Each wave walks
  * its 8 rows with the masks of the five rows around the current one: a pixel
  * whose 5 x 5 window holds no flagged pixel (nearly all) gets the identity,
  * the others add the terms of the flagged pixels of the window, whose bits
