@@ -16,6 +16,7 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <thread>
 #include <new>
 #include <stdexcept>
 #include <vector>
@@ -110,14 +111,40 @@ check_arguments(const cusmic_image *ims, size_t nf, const cusmic_options *o, con
 	return total;
 }
 
-/* Host map contents (src/api.cu:107-114). */
+/* Host map contents (src/api.cu:107-114), split over host threads for large
+ * maps: the check reads every value once, at the host's memory bandwidth. */
 void
 check_values(const double *data, size_t n, const char *name, int minimum)
 {
-	for (size_t i = 0; data && i < n; ++i)
-		if (!std::isfinite(data[i]) || (minimum == 1 && data[i] <= 0) ||
-			(minimum == 0 && data[i] < 0))
-			throw std::invalid_argument(name);
+	if (!data)
+		return;
+	auto bad_in = [=](size_t lo, size_t hi) {
+		bool bad = false;
+		for (size_t i = lo; i < hi; ++i)
+			bad |= !std::isfinite(data[i]) || (minimum == 1 && data[i] <= 0) ||
+			       (minimum == 0 && data[i] < 0);
+		return bad;
+	};
+	constexpr size_t PER_THREAD = size_t(1) << 20;
+	const size_t threads = std::min<size_t>({16, std::max(1u, std::thread::hardware_concurrency()),
+		(n + PER_THREAD - 1) / PER_THREAD});
+	bool bad = false;
+	if (threads <= 1) {
+		bad = bad_in(0, n);
+	} else {
+		std::vector<char> part(threads, 0);
+		std::vector<std::thread> pool;
+		const size_t step = (n + threads - 1) / threads;
+		for (size_t t = 1; t < threads; ++t)
+			pool.emplace_back([&, t] { part[t] = bad_in(std::min(n, t * step), std::min(n, (t + 1) * step)); });
+		part[0] = bad_in(0, std::min(n, step));
+		for (auto &th : pool)
+			th.join();
+		for (char b : part)
+			bad |= b != 0;
+	}
+	if (bad)
+		throw std::invalid_argument(name);
 }
 
 /* Stream-ordered scratch comes from one memory pool per device that keeps
@@ -603,36 +630,41 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 			passes[f] = detect_passes(f);
 }
 
-struct host_result {
-	std::unique_ptr<double[]> clean;
-	std::unique_ptr<uint8_t[]> mask;
-	explicit host_result(size_t n) : clean(new double[n]), mask(new uint8_t[n]) {}
-};
-
-/* Host path (src/api.cu:116-153): maxiter = 0 stays on the CPU. */
-host_result
-clean_host(const cusmic_image &im, const cusmic_options &o, size_t n)
+/* Host path (src/api.cu:116-153): the maps' values are checked first, on the
+ * host; maxiter = 0 stays on the CPU; otherwise the planes are uploaded and
+ * the results copied straight into the caller's buffers once the call has
+ * succeeded, so that a failed call leaves them untouched. */
+void
+clean_host(const cusmic_image &im, const cusmic_options &o, size_t n, double *clean_out,
+	uint8_t *mask_out)
 {
-	host_result out(n);
+	if (im.error)
+		check_values(im.error, n, "error must be finite and positive", 1);
+	else {
+		check_values(im.gain, n, "gain must be finite and positive", 1);
+		check_values(im.readnoise, n, "read noise must be finite and nonnegative", 0);
+	}
+	check_values(im.background, n, "background must be finite", -1);
 	if (!o.maxiter) {
-		std::memcpy(out.clean.get(), im.data, n * sizeof(double));
-		std::memset(out.mask.get(), 0, n);
+		if (clean_out != im.data)
+			std::memmove(clean_out, im.data, n * sizeof(double));
+		std::memset(mask_out, 0, n);
 		if (im.background || !std::isnan(o.background))
 			for (size_t i = 0; i < n; ++i) {
-				if (!std::isfinite(im.data[i]))
+				if (!std::isfinite(clean_out[i]))
 					continue;
 				double bg = im.background ? im.background[i] : o.background;
-				out.clean[i] += bg;
-				out.clean[i] -= bg;
+				clean_out[i] += bg;
+				clean_out[i] -= bg;
 			}
-		return out;
+		return;
 	}
 	hipStream_t stream = nullptr;
 	const double *gmap = im.error ? nullptr : im.gain;
 	const double *rnmap = im.error ? nullptr : im.readnoise;
 	size_t planes = 2 + (im.error != nullptr) + (gmap != nullptr) + (rnmap != nullptr) +
 			(im.background != nullptr);
-	device_block mem(planes * n * sizeof(double) + 2 * n + 512, stream);
+	device_block mem(planes * n * sizeof(double) + 2 * align_up(n, 256) + 512, stream);
 	char *base = static_cast<char *>(mem.ptr);
 	double *d = reinterpret_cast<double *>(base);
 	double *next = d;
@@ -662,11 +694,9 @@ clean_host(const cusmic_image &im, const cusmic_options &o, size_t n)
 	gpu.height = im.height;
 	clean_images(&gpu, 1, o, nullptr, nullptr, CUSMIC_LAPLACIAN_EXACT, clean, mask, nullptr,
 		nullptr, stream);
-	hip_check(hipMemcpyAsync(out.clean.get(), clean, n * sizeof(double), hipMemcpyDeviceToHost,
-		stream));
-	hip_check(hipMemcpyAsync(out.mask.get(), mask, n, hipMemcpyDeviceToHost, stream));
+	hip_check(hipMemcpyAsync(clean_out, clean, n * sizeof(double), hipMemcpyDeviceToHost, stream));
+	hip_check(hipMemcpyAsync(mask_out, mask, n, hipMemcpyDeviceToHost, stream));
 	hip_check(hipStreamSynchronize(stream));
-	return out;
 }
 
 } // namespace
@@ -696,16 +726,7 @@ remove_cosmics(const cusmic_image *im, const cusmic_options *o, double *clean, u
 {
 	return guarded([&] {
 		size_t n = check_arguments(im, 1, o, clean, mask, true);
-		if (im->error)
-			check_values(im->error, n, "error must be finite and positive", 1);
-		else {
-			check_values(im->gain, n, "gain must be finite and positive", 1);
-			check_values(im->readnoise, n, "read noise must be finite and nonnegative", 0);
-		}
-		check_values(im->background, n, "background must be finite", -1);
-		auto out = clean_host(*im, *o, n);
-		std::memcpy(clean, out.clean.get(), n * sizeof(double));
-		std::memcpy(mask, out.mask.get(), n);
+		clean_host(*im, *o, n, clean, mask);
 	}, msg, len);
 }
 

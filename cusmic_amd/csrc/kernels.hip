@@ -307,8 +307,18 @@ epilogue(Params p, long long total, int bg_maps)
 			for (int k = 0; k < 8 && g0 + k < total; ++k)
 				p.mask[g0 + k] = uint8_t(m >> (8 * k));
 		}
-		if (!__syncthreads_or((fl & LOW) != 0 || bg_maps || bg_scalar))
+		const bool holes = __syncthreads_or((fl & LOW) != 0);
+		if (!holes && !bg_maps) {
+			/* no frame lookup: every pixel of the step loses the scalar */
+			if (bg_scalar) {
+				for (int k = 0; k < 8; ++k) {
+					const long long g = b0 + k * BLOCK + threadIdx.x;
+					if (g < total)
+						p.clean[g] = p.clean[g] + -1 * p.bg;
+				}
+			}
 			continue;
+		}
 		int f = 0;
 		{
 			const long long g = min(b0 + threadIdx.x, total - 1);
