@@ -24,7 +24,10 @@ the timed steps:
     of the batch, checked against the GPU's output for that frame;
     cpu_baseline_all_cores: the same oracle on every host core on four frames.
 Prints one JSON line on stdout; progress goes to stderr. ``--only LEG``
-(detector or fwd-bwd) runs one leg alone, for profiling.
+(detector or fwd-bwd) runs one leg alone, for profiling. ``--record DIR``
+instead writes records in the reference's schema (bench/bench.cu:297-332) for
+its saved scene stacked 1, 4 and 16 times, and ``--ab BASELINE CANDIDATE``
+compares two such directories under the reference's A/B rules (bench/ab.py).
 """
 
 import argparse
@@ -397,14 +400,30 @@ def main():
     ap.add_argument("--no-host", action="store_true")
     ap.add_argument("--only", choices=("detector", "fwd-bwd"),
                     help="run this leg alone (for profiling) and print its record")
+    ap.add_argument("--record", type=Path, metavar="DIR",
+                    help="write benchmark records in the reference's schema (its saved scene, stacks of "
+                         "1, 4 and 16 frames) into DIR, and exit")
+    ap.add_argument("--ab", nargs=2, type=Path, metavar=("BASELINE", "CANDIDATE"),
+                    help="compare two record directories with the reference's A/B rules, and exit")
     ap.add_argument("--calibration-copy", action="store_true",
                     help="start with a 1 GiB device-to-device copy, whose hardware counter "
                          "readings scale FETCH_SIZE/WRITE_SIZE to bytes")
     args = ap.parse_args()
+    if args.ab:
+        from cusmic_amd import bench_record
+        try:
+            print("\n".join(bench_record.report(*args.ab)))
+        except (OSError, ValueError, KeyError, TypeError, json.JSONDecodeError) as exc:
+            ap.error(str(exc))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only; on several GPUs the frames of a batch would be "
               "sharded by pixel count, with no exchange between GPUs", file=sys.stderr)
         sys.exit(2)
+    if args.record:
+        from cusmic_amd import bench_record
+        bench_record.record(args.record)
+        return
     b = Bench(args)
     if args.calibration_copy:
         a = b.torch.ones(1 << 27, dtype=b.torch.float64, device=b.dev)
