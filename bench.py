@@ -42,25 +42,27 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "cleaned Mpix/s fp64 maxiter=4: ragged-batch vs per-frame; fwd+bwd Mpix/s"
 HBM_PEAK_GBS = 8000.0
-# first detection pass, per frame pixel: reads the data and the error map,
-# writes the working image and the flags (plus the input mask where there is
-# one)
-FIRST_BYTES_PER_PX = 8.0 + 8.0 + 8.0 + 1.0
+# first detection pass, per frame pixel: reads the data and the error map and
+# writes the working image (the flags plane is zeroed by a streaming fill
+# kernel; the pass writes flags and S' only at the ~1% of pixels it screens),
+# plus a read of the input mask where there is one
+FIRST_BYTES_PER_PX = 8.0 + 8.0 + 8.0
 TRAFFIC_FILE = ROOT / "profiles" / "significance_maps_ragged64_traffic.json"
+DETECTOR_TRAFFIC_FILE = ROOT / "profiles" / "significance_maps_detector4096_traffic.json"
 SAVED_SCENE = ROOT / "tests" / "golden" / "fits"
 
 
-def significance_traffic(launches):
+def significance_traffic(launches, path=TRAFFIC_FILE):
     """HBM bytes per first-pass launch from the committed counter summary
     (FETCH_SIZE + WRITE_SIZE of the same workload, scaled by a copy kernel of
     known size), or None when there is none."""
     try:
-        rec = json.loads(TRAFFIC_FILE.read_text())
+        rec = json.loads(path.read_text())
     except (OSError, ValueError):
         return None, None
     if rec.get("launches") != launches:
         return None, None
-    return round(rec["hbm_bytes_per_launch"]), TRAFFIC_FILE.name
+    return round(rec["hbm_bytes_per_launch"]), path.name
 
 
 def log(*a):
@@ -230,11 +232,12 @@ class Bench:
         ms = float(np.median(samples))
         first = float(np.median([p["first_pass_ms"] for p in profiles]))
         log(f"detector: {ms:.3f} ms per call, first pass {first:.3f} ms")
-        # first pass per pixel: data, error map, working image, flags and the
-        # input mask (the scalar background costs no bytes)
+        # first pass per pixel: data, error map and input mask read, working
+        # image written (the scalar background costs no bytes)
+        traffic, source = significance_traffic(1, DETECTOR_TRAFFIC_FILE)
         return {"value": round(n / 1e6 / (ms / 1e3), 2), "unit": "Mpix/s", "ms_per_call": round(ms, 4),
                 "frame": "4096x4096, error map, background 150, 0.5% masked", "flagged_pixels": int(mask.sum()),
-                "first_pass_roofline": roofline((FIRST_BYTES_PER_PX + 1.0) * n, first)}
+                "first_pass_roofline": roofline((FIRST_BYTES_PER_PX + 1.0) * n, first, 1, traffic, source)}
 
     # -------------------------------------------------------------- host
     def host(self):
