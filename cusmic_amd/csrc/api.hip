@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <condition_variable>
 #include <mutex>
 #include <thread>
 #include <new>
@@ -237,6 +238,165 @@ device_cus()
 	hip_check(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
 	return n > 0 ? n : 1;
 }
+
+/* Host copies split over a few persistent threads: pageable memory moves to
+ * and from pinned staging at several times one thread's memcpy rate. The
+ * caller's thread takes the first slice. */
+class copy_threads {
+public:
+	static constexpr unsigned MAX_THREADS = 8;
+
+	static copy_threads &get()
+	{
+		/* never destroyed: its workers stay parked until the process ends */
+		static copy_threads *pool = new copy_threads;
+		return *pool;
+	}
+
+	void copy(void *dst, const void *src, size_t bytes)
+	{
+		const size_t slices = std::min<size_t>(workers_.size() + 1, std::max<size_t>(1, bytes >> 20));
+		if (slices <= 1) {
+			std::memcpy(dst, src, bytes);
+			return;
+		}
+		std::lock_guard<std::mutex> one_job(job_lock_);
+		const size_t part = (bytes + slices - 1) / slices;
+		{
+			std::lock_guard<std::mutex> l(m_);
+			dst_ = static_cast<char *>(dst);
+			src_ = static_cast<const char *>(src);
+			bytes_ = bytes;
+			part_ = part;
+			slices_ = slices;
+			pending_ = slices - 1;
+			++generation_;
+		}
+		go_.notify_all();
+		std::memcpy(dst, src, std::min(part, bytes));
+		std::unique_lock<std::mutex> l(m_);
+		done_.wait(l, [&] { return pending_ == 0; });
+	}
+
+private:
+	copy_threads()
+	{
+		const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+		const unsigned n = std::min(MAX_THREADS, hw) - 1;
+		for (unsigned k = 0; k < n; ++k)
+			workers_.emplace_back([this, k] { work(k + 1); });
+	}
+
+	void work(size_t slice)
+	{
+		unsigned seen = 0;
+		for (;;) {
+			char *dst;
+			const char *src;
+			size_t lo, hi;
+			{
+				std::unique_lock<std::mutex> l(m_);
+				go_.wait(l, [&] { return generation_ != seen; });
+				seen = generation_;
+				if (slice >= slices_)
+					continue;
+				dst = dst_;
+				src = src_;
+				lo = std::min(bytes_, slice * part_);
+				hi = std::min(bytes_, lo + part_);
+			}
+			std::memcpy(dst + lo, src + lo, hi - lo);
+			std::lock_guard<std::mutex> l(m_);
+			if (--pending_ == 0)
+				done_.notify_one();
+		}
+	}
+
+	std::vector<std::thread> workers_;
+	std::mutex job_lock_, m_;
+	std::condition_variable go_, done_;
+	char *dst_ = nullptr;
+	const char *src_ = nullptr;
+	size_t bytes_ = 0, part_ = 0, slices_ = 0, pending_ = 0;
+	unsigned generation_ = 0;
+};
+
+/* Copies between pageable host memory and the device through two pinned
+ * staging buffers: the host copy of one chunk overlaps the DMA of the other.
+ * Copies below DIRECT bytes go straight from pageable memory, where the
+ * hand-off would cost more than it saves. One staging set per process; calls
+ * take turns on it. */
+class staging {
+public:
+	static constexpr size_t CHUNK = size_t(32) << 20;
+	static constexpr size_t DIRECT = size_t(4) << 20;
+
+	static staging &get()
+	{
+		static staging *s = new staging;
+		return *s;
+	}
+
+	std::mutex lock;
+
+	void upload(void *dst, const void *src, size_t bytes, hipStream_t stream)
+	{
+		if (bytes < DIRECT) {
+			hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
+			return;
+		}
+		ensure();
+		char *d = static_cast<char *>(dst);
+		const char *s = static_cast<const char *>(src);
+		for (size_t off = 0, k = 0; off < bytes; off += CHUNK, ++k) {
+			const size_t len = std::min(CHUNK, bytes - off), b = k & 1;
+			hip_check(hipEventSynchronize(done_[b]));
+			copy_threads::get().copy(pin_[b], s + off, len);
+			hip_check(hipMemcpyAsync(d + off, pin_[b], len, hipMemcpyHostToDevice, stream));
+			hip_check(hipEventRecord(done_[b], stream));
+		}
+	}
+
+	void download(void *dst, const void *src, size_t bytes, hipStream_t stream)
+	{
+		if (bytes < DIRECT) {
+			hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));
+			return;
+		}
+		ensure();
+		char *d = static_cast<char *>(dst);
+		const char *s = static_cast<const char *>(src);
+		const size_t chunks = (bytes + CHUNK - 1) / CHUNK;
+		auto fetch = [&](size_t k) {
+			const size_t off = k * CHUNK, len = std::min(CHUNK, bytes - off), b = k & 1;
+			hip_check(hipMemcpyAsync(pin_[b], s + off, len, hipMemcpyDeviceToHost, stream));
+			hip_check(hipEventRecord(done_[b], stream));
+		};
+		for (size_t k = 0; k < std::min<size_t>(2, chunks); ++k)
+			fetch(k);
+		for (size_t k = 0; k < chunks; ++k) {
+			const size_t off = k * CHUNK, len = std::min(CHUNK, bytes - off), b = k & 1;
+			hip_check(hipEventSynchronize(done_[b]));
+			copy_threads::get().copy(d + off, pin_[b], len);
+			if (k + 2 < chunks)
+				fetch(k + 2);
+		}
+	}
+
+private:
+	void ensure()
+	{
+		if (pin_[0])
+			return;
+		for (int b = 0; b < 2; ++b) {
+			hip_check(hipHostMalloc(&pin_[b], CHUNK, hipHostMallocDefault));
+			hip_check(hipEventCreateWithFlags(&done_[b], hipEventDisableTiming));
+		}
+	}
+
+	void *pin_[2] = {nullptr, nullptr};
+	hipEvent_t done_[2] = {nullptr, nullptr};
+};
 
 /* Frame table of a batch: offsets and tile numbering. */
 std::vector<Frame>
@@ -633,7 +793,8 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 /* Host path (src/api.cu:116-153): the maps' values are checked first, on the
  * host; maxiter = 0 stays on the CPU; otherwise the planes are uploaded and
  * the results copied straight into the caller's buffers once the call has
- * succeeded, so that a failed call leaves them untouched. */
+ * succeeded, so that a failed call leaves them untouched. Both directions go
+ * through the pinned staging buffers. */
 void
 clean_host(const cusmic_image &im, const cusmic_options &o, size_t n, double *clean_out,
 	uint8_t *mask_out)
@@ -660,6 +821,8 @@ clean_host(const cusmic_image &im, const cusmic_options &o, size_t n, double *cl
 		return;
 	}
 	hipStream_t stream = nullptr;
+	staging &stage = staging::get();
+	std::lock_guard<std::mutex> staged(stage.lock);
 	const double *gmap = im.error ? nullptr : im.gain;
 	const double *rnmap = im.error ? nullptr : im.readnoise;
 	size_t planes = 2 + (im.error != nullptr) + (gmap != nullptr) + (rnmap != nullptr) +
@@ -673,7 +836,7 @@ clean_host(const cusmic_image &im, const cusmic_options &o, size_t n, double *cl
 			return nullptr;
 		double *dst = next;
 		next += n;
-		hip_check(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, stream));
+		stage.upload(dst, src, n * sizeof(double), stream);
 		return dst;
 	};
 	cusmic_image gpu{};
@@ -687,15 +850,15 @@ clean_host(const cusmic_image &im, const cusmic_options &o, size_t n, double *cl
 	uint8_t *mask = reinterpret_cast<uint8_t *>(next);
 	uint8_t *excl = mask + align_up(n, 256);
 	if (im.mask) {
-		hip_check(hipMemcpyAsync(excl, im.mask, n, hipMemcpyHostToDevice, stream));
+		stage.upload(excl, im.mask, n, stream);
 		gpu.mask = excl;
 	}
 	gpu.width = im.width;
 	gpu.height = im.height;
 	clean_images(&gpu, 1, o, nullptr, nullptr, CUSMIC_LAPLACIAN_EXACT, clean, mask, nullptr,
 		nullptr, stream);
-	hip_check(hipMemcpyAsync(clean_out, clean, n * sizeof(double), hipMemcpyDeviceToHost, stream));
-	hip_check(hipMemcpyAsync(mask_out, mask, n, hipMemcpyDeviceToHost, stream));
+	stage.download(clean_out, clean, n * sizeof(double), stream);
+	stage.download(mask_out, mask, n, stream);
 	hip_check(hipStreamSynchronize(stream));
 }
 
