@@ -1310,8 +1310,9 @@ constexpr int SEED_BUF = 512; /* seeds buffered per workgroup between flushes */
 struct SigShared {
 	double s[RING][SW];              /* S, row r of a run in slot r % RING */
 	unsigned long long screen[RING]; /* per row: tile columns with S > min(t, t_n) */
-	int bad[RING];                   /* per row: bit 0 an invalid noise value,
-					  * bit 1 a hole among the samples of C read */
+	int step_bits[2][BLOCK / 64];    /* per step parity and wave: bit 0 an invalid
+					  * noise value among its rows, bit 1 a hole
+					  * among the samples of C they read */
 	uint16_t alist[TILE_W * TILE_H]; /* screened pixels of the tile: row << 6 | lane */
 	unsigned long long seeds[SEED_BUF];
 	int na, nseed, item;
@@ -1319,10 +1320,10 @@ struct SigShared {
 	int own[3];                      /* the run's holes, allowed pixels, donors */
 };
 
-/* This lane's counts of the pixels of a run it owns. */
+/* This lane's counts of the pixels of a run it owns (the run's pixels less
+ * its holes and masked pixels are the allowed ones). */
 struct OwnCounts {
 	int holes;            /* holes (skipped tiles) */
-	int nonhole;          /* pixels that are no hole */
 	int masked;           /* masked pixels that are no hole */
 	int nonfinite;        /* pixels that are no hole with a nonfinite C */
 	int masked_nonfinite; /* masked ones of those */
@@ -1357,35 +1358,41 @@ flush_seeds(SigShared &sh, unsigned long long *list, unsigned int *count)
 		sh.nseed = 0;
 }
 
-/* Store one row of S (this lane's slot) and its screen, noise and hole flags. */
+/* Store one row of S (this lane's slot) and its screen. */
 __device__ __forceinline__ void
-emit_row(SigShared &sh, int &slot, int lane, double t_min, unsigned long long cols, double v, bool bad,
-	bool hole)
+emit_row(SigShared &sh, int &slot, int lane, double t_min, unsigned long long cols, double v)
 {
 	sh.s[slot][lane] = v;
 	const unsigned long long m = __ballot(v > t_min) & cols;
-	const int rowbits = int(__ballot(bad) != 0) | int(__ballot(hole) != 0) << 1;
-	/* every lane stores the same words: without a branch the loads of later
+	/* every lane stores the same word: without a branch the loads of later
 	 * rows can be issued ahead of this one */
 	sh.screen[slot] = m;
-	sh.bad[slot] = rowbits;
 	slot = slot + 1 == RING ? 0 : slot + 1;
 }
 
+/* The noise and hole flags of one wave's rows of a step: one word, written
+ * once after its rows (flags: this lane saw an invalid noise value / a hole). */
+__device__ __forceinline__ void
+emit_step_bits(int *word, int lane, bool bad, bool hole)
+{
+	const int bits = int(__ballot(bad) != 0) | int(__ballot(hole) != 0) << 1;
+	if (!lane)
+		*word = bits;
+}
+
 /* The prologue's work at a row of owned pixels (own: this lane's pixel i
- * belongs to the run): C (unless the caller stores it later) and the counts
- * behind the per-frame allowed pixels and donors. Branch-free. Masked pixels
- * are flagged and counted per tile, holes where their tile skips. */
+ * belongs to the run): C (unless the caller stores it later) and, with a
+ * background, the counts of nonfinite C behind the donors (a hole's C is the
+ * finite background). Branch-free. Masked pixels are flagged and counted per
+ * tile, holes where their tile skips. */
 template <bool BG>
 __device__ __forceinline__ void
-own_row(const Params &p, const Frame &fr, bool own, long long i, double c, bool hole, bool store_c,
-	OwnCounts &acc)
+own_row(const Params &p, const Frame &fr, bool own, long long i, double c, bool store_c, OwnCounts &acc)
 {
 	if (store_c && own)
 		p.clean[fr.off + i] = c;
-	acc.nonhole += own && !hole;
 	if (BG) {
-		acc.nonfinite += own && !hole && !isfinite(c);
+		acc.nonfinite += own && !isfinite(c);
 		acc.infinite |= own && isinf(c);
 	}
 }
@@ -1446,7 +1453,7 @@ template <bool EXACT, bool BG>
 __device__ __forceinline__ void
 model_row_region(SigShared &sh, int &slot, int lane, double t_min, unsigned long long cols,
 	const Params &p, const Frame &fr, int gx, int gy, const double *rows, int k, double med, bool hm,
-	bool hc, bool own, OwnCounts &acc)
+	bool hc, bool own, OwnCounts &acc, bool &any_bad, bool &any_hole)
 {
 	const int mode = p.mode, w = fr.w, h = fr.h;
 	const double cc = rows[(k + 2) * 5 + 2];
@@ -1463,12 +1470,13 @@ model_row_region(SigShared &sh, int &slot, int lane, double t_min, unsigned long
 	const double gg = fr.gain ? fr.gain[mi] : fr.g;
 	const double rn = fr.readnoise ? fr.readnoise[mi] : fr.rn;
 	const double nz = noise_model(med, gg, rn);
-	own_row<BG>(p, fr, own, (long long)gy * w + gx, cc, hc, true, acc);
+	own_row<BG>(p, fr, own, (long long)gy * w + gx, cc, true, acc);
 	/* lanes 0 and 63 carry C only: their S is never read */
 	const bool s_lane = lane >= 1 && lane <= SW - 2;
 	emit_row(sh, slot, lane, t_min, cols,
-		laplacian_q<EXACT, !BG>(cc * 0.25, u * 0.25, d * 0.25, l * 0.25, r * 0.25, w == 1) / (2 * nz),
-		s_lane && (in || mode == CUSMIC_WRAP) && invalid_noise(nz), hm);
+		laplacian_q<EXACT, !BG>(cc * 0.25, u * 0.25, d * 0.25, l * 0.25, r * 0.25, w == 1) / (2 * nz));
+	any_bad |= s_lane && (in || mode == CUSMIC_WRAP) && invalid_noise(nz);
+	any_hole |= hm;
 }
 
 /* One wave: S of run rows [r0, r0 + NR) into the ring (run row r is frame row
@@ -1492,7 +1500,7 @@ template <bool MODEL, bool EXACT, bool BG, bool BGMAP, int NR>
 __device__ __forceinline__ void
 significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0, int yb, int r0,
 	double t_min, unsigned long long cols, int own0, int own1, OwnCounts &acc,
-	const ModelRegion *R, int rS0)
+	const ModelRegion *R, int rS0, int *step_word)
 {
 	constexpr bool TAME = !BG;
 	/* an opaque lane keeps the compiler from hoisting this step's per-lane
@@ -1508,6 +1516,7 @@ significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0
 	const bool s_lane = lane >= 1 && lane <= SW - 2;
 	const int gx = X0 - HS - 1 + lane, ya = yb + r0;
 	int slot = r0 % RING;
+	bool any_bad = false, any_hole = false;
 	/* the rows and their one-pixel ring lie inside the frame */
 	const bool inner = X0 - HS - 1 >= 0 && X0 - HS - 1 + SW <= w && ya - 1 >= 0 && ya + nr < h;
 	if ((!MODEL || fr.error) && inner) {
@@ -1530,9 +1539,9 @@ significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0
 		if (BG && bp)                                                              \
 			bn[k] = bp[ro + W];                                                \
 	} while (0)
-		bool hu = false, hc = false;
+		bool hu = false, hc0 = false;
 		double qu = input_c<BG>(dp[-W], bp ? bp[-W] : p.bg, has_bg, hu) * 0.25;
-		double cc = input_c<BG>(dp[0], bp ? bp[0] : p.bg, has_bg, hc);
+		double cc = input_c<BG>(dp[0], bp ? bp[0] : p.bg, has_bg, hc0);
 		double qc = cc * 0.25;
 #pragma unroll
 		for (int r = 0; r < PF && r < NR; ++r)
@@ -1555,16 +1564,18 @@ significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0
 			const bool own = own_lane && gy >= own0 && gy < own1;
 			if (own)
 				__builtin_nontemporal_store(cc, &p.clean[fr.off + o + r * W]);
-			own_row<BG>(p, fr, own, o + r * W, cc, hc, false, acc);
+			own_row<BG>(p, fr, own, o + r * W, cc, false, acc);
 			emit_row(sh, slot, lane, t_min, cols,
-				laplacian_q<EXACT, TAME>(qc, qu, qd, ql, qr, false) / (2 * nz[r]),
-				s_lane && invalid_noise(nz[r]), hc || (s_lane && (hu || hd)));
+				laplacian_q<EXACT, TAME>(qc, qu, qd, ql, qr, false) / (2 * nz[r]));
+			any_bad |= s_lane && invalid_noise(nz[r]);
+			any_hole |= hd;
 			qu = qc;
 			qc = qd;
 			cc = cd;
-			hu = hc;
-			hc = hd;
 		}
+		/* every sample of C this lane loaded (lanes 0 and 63: also the rows
+		 * above and below theirs, which only makes the flag cautious) */
+		any_hole |= hu || hc0;
 #undef FETCH_ROW
 	} else if (!MODEL || fr.error) {
 		bool hu = false, hc = false;
@@ -1579,11 +1590,11 @@ significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0
 			const double qr = lap_input<BG>(fr, p.bg, has_bg, gx + 1, gy, mode, hs) * 0.25;
 			const double nz = fr.error[map_index(gx, gy, w, h, mode)];
 			const bool in = gx >= 0 && gx < w && gy >= 0 && gy < h;
-			own_row<BG>(p, fr, own_lane && gy >= own0 && gy < own1, (long long)gy * w + gx, cc, hc, true,
-				acc);
+			own_row<BG>(p, fr, own_lane && gy >= own0 && gy < own1, (long long)gy * w + gx, cc, true, acc);
 			emit_row(sh, slot, lane, t_min, cols,
-				laplacian_q<EXACT, TAME>(cc * 0.25, qu, cd * 0.25, ql, qr, narrow) / (2 * nz),
-				s_lane && (in || wrap) && invalid_noise(nz), hc || (s_lane && (hu || hd || hs)));
+				laplacian_q<EXACT, TAME>(cc * 0.25, qu, cd * 0.25, ql, qr, narrow) / (2 * nz));
+			any_bad |= s_lane && (in || wrap) && invalid_noise(nz);
+			any_hole |= hc || (s_lane && (hu || hd || hs));
 			qu = cc * 0.25;
 			cc = cd;
 			hu = hc;
@@ -1615,10 +1626,11 @@ significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0
 				const bool hc = lane + HS < 64 ? (R->hole_lo[kc] >> (lane + HS)) & 1
 							      : (R->hole_hi[kc] >> (lane + HS - 64)) & 1;
 				model_row_region<EXACT, BG>(sh, slot, lane, t_min, cols, p, fr, gx, gy, rows, k, med[k],
-					hm, hc, own_lane && gy >= own0 && gy < own1, acc);
+					hm, hc, own_lane && gy >= own0 && gy < own1, acc, any_bad, any_hole);
 			}
 		}
 	}
+	emit_step_bits(step_word, lane, any_bad, any_hole);
 }
 
 /* The first detection pass, which also does the prologue's work (C, flags,
@@ -1667,7 +1679,7 @@ significance_pass(SigShared &sh, ModelRegion *region, const Params &p, int iter)
 		const int ncols = min(TILE_W, w - X0), own1 = min(Y0 + TILE_H * K, h);
 		const unsigned long long cols = ((1ull << ncols) - 1) << (HS + 1);
 		const bool has_bg = BG && (fr.background || !isnan(p.bg));
-		OwnCounts acc{0, 0, 0, 0, 0, false};
+		OwnCounts acc{0, 0, 0, 0, false};
 
 		for (int s = 0; s < K; ++s) {
 			/* per-step opaque thread index, as in significance_rows */
@@ -1688,11 +1700,11 @@ significance_pass(SigShared &sh, ModelRegion *region, const Params &p, int iter)
 			if (s == 0) {
 				const int r0 = wave * (STEP_ROWS / 4);
 				significance_rows<MODEL, EXACT, BG, BGMAP, STEP_ROWS / 4>(sh, p, fr, f, X0, yb, r0, t_min,
-					cols, Y0, own1, acc, region, rS0);
+					cols, Y0, own1, acc, region, rS0, &sh.step_bits[s & 1][wave]);
 			} else {
 				const int r0 = TILE_H * s + 2 * HS + wave * (TILE_H / 4);
 				significance_rows<MODEL, EXACT, BG, BGMAP, TILE_H / 4>(sh, p, fr, f, X0, yb, r0, t_min,
-					cols, Y0, own1, acc, region, rS0);
+					cols, Y0, own1, acc, region, rS0, &sh.step_bits[s & 1][wave]);
 			}
 			const int ty = Y0 + TILE_H * s; /* first frame row of the tile */
 			const int nrows = min(TILE_H, h - ty);
@@ -1724,8 +1736,16 @@ significance_pass(SigShared &sh, ModelRegion *region, const Params &p, int iter)
 			 * the medians read, every pixel is kept. With a hole among the
 			 * samples, the tile's blocks are left to the block pass after the
 			 * fill. */
-			const int rb = lane < STEP_ROWS ? sh.bad[(r_lo + lane) % RING] : 0;
-			const bool unsafe = __ballot(rb & 1) != 0, skip = __ballot(rb & 2) != 0;
+			/* the tile's medians read rows [r_lo, r_lo + STEP_ROWS): this
+			 * step's rows and the last four of the previous step, which its
+			 * last wave computed */
+			int rb = 0;
+#pragma unroll
+			for (int k = 0; k < BLOCK / 64; ++k)
+				rb |= sh.step_bits[s & 1][k];
+			if (s > 0)
+				rb |= sh.step_bits[(s - 1) & 1][BLOCK / 64 - 1];
+			const bool unsafe = (rb & 1) != 0, skip = (rb & 2) != 0;
 			if (skip) {
 				const int bx0 = X0 / BLK, by0 = ty / BLK;
 				const int nbx = (X0 + ncols - 1) / BLK - bx0 + 1;
@@ -1832,7 +1852,10 @@ significance_pass(SigShared &sh, ModelRegion *region, const Params &p, int iter)
 		/* the run's counts: one LDS atomic per wave, one global atomic per
 		 * count and run */
 		{
-			const int allowed = acc.nonhole - acc.masked;
+			/* the run's own pixels, counted once, less this lane's holes and
+			 * masked pixels */
+			const int own_total = !threadIdx.x ? ncols * (own1 - Y0) : 0;
+			const int allowed = own_total - acc.holes - acc.masked;
 			int c3[3] = {acc.holes, allowed, allowed - (BG ? acc.nonfinite - acc.masked_nonfinite : 0)};
 #pragma unroll
 			for (int k = 0; k < 3; ++k) {
