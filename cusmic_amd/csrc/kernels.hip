@@ -1287,21 +1287,22 @@ med_input(const Frame &fr, double bg, bool has_bg, int x, int y, int mode, bool 
 	return bx < 0 || by < 0 ? 0.0 : input_at<BG>(fr, bg, has_bg, (long long)by * fr.w + bx, hole);
 }
 
-/* Lane i takes lane i - 1's value (wave_shr:1); lane 0 keeps fill. */
+/* Lane i takes lane i - 1's value (wave_shr:1); lane 0 gets 0 (bound_ctrl,
+ * so no register has to hold an old value). */
 __device__ __forceinline__ double
-wave_shift_right(double v, double fill)
+wave_shift_right(double v)
 {
-	const int lo = __builtin_amdgcn_update_dpp(__double2loint(fill), __double2loint(v), 0x138, 0xf, 0xf, false);
-	const int hi = __builtin_amdgcn_update_dpp(__double2hiint(fill), __double2hiint(v), 0x138, 0xf, 0xf, false);
+	const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x138, 0xf, 0xf, true);
+	const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x138, 0xf, 0xf, true);
 	return __hiloint2double(hi, lo);
 }
 
-/* Lane i takes lane i + 1's value (wave_shl:1); lane 63 keeps fill. */
+/* Lane i takes lane i + 1's value (wave_shl:1); lane 63 gets 0. */
 __device__ __forceinline__ double
-wave_shift_left(double v, double fill)
+wave_shift_left(double v)
 {
-	const int lo = __builtin_amdgcn_update_dpp(__double2loint(fill), __double2loint(v), 0x130, 0xf, 0xf, false);
-	const int hi = __builtin_amdgcn_update_dpp(__double2hiint(fill), __double2hiint(v), 0x130, 0xf, 0xf, false);
+	const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x130, 0xf, 0xf, true);
+	const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x130, 0xf, 0xf, true);
 	return __hiloint2double(hi, lo);
 }
 
@@ -1522,26 +1523,32 @@ significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0
 	if ((!MODEL || fr.error) && inner) {
 		/* global-space pointers: flat loads would also count against the
 		 * LDS wait counter */
+		using gchar = const __attribute__((address_space(1))) char;
 		using gdouble = const __attribute__((address_space(1))) double;
-		const long long o = (long long)ya * w + gx, W = w;
-		gdouble *dp = (gdouble *)(fr.data + o), *ep = (gdouble *)(fr.error + o);
-		gdouble *bp = BGMAP && fr.background ? (gdouble *)(fr.background + o) : nullptr;
+		/* addresses: a uniform row base plus the lane's byte offset, so that
+		 * the loads and stores take a scalar base */
+		const long long rb = (long long)ya * w + (X0 - HS - 1), W = w;
+		const unsigned lane8 = unsigned(lane) * 8u;
+		auto at = [lane8](const double *base, long long idx) -> double {
+			return *(gdouble *)((gchar *)(base + idx) + lane8);
+		};
+		const double *bp = BGMAP && fr.background ? fr.background : nullptr;
 		/* the samples of row r: the pixel below and the noise (and the
 		 * background below), fetched PF rows ahead */
 		constexpr int PF = BGMAP ? 4 : 5;
 		double dn[NR], nz[NR], bn[NR];
 #define FETCH_ROW(k)                                                                       \
 	do {                                                                               \
-		const long long ro = (long long)(k) * W;                                   \
-		dn[k] = dp[ro + W];                                                        \
-		nz[k] = ep[ro];                                                            \
+		const long long ro = rb + (long long)(k) * W;                              \
+		dn[k] = at(fr.data, ro + W);                                               \
+		nz[k] = at(fr.error, ro);                                                  \
 		bn[k] = p.bg;                                                              \
 		if (BG && bp)                                                              \
-			bn[k] = bp[ro + W];                                                \
+			bn[k] = at(bp, ro + W);                                            \
 	} while (0)
 		bool hu = false, hc0 = false;
-		double qu = input_c<BG>(dp[-W], bp ? bp[-W] : p.bg, has_bg, hu) * 0.25;
-		double cc = input_c<BG>(dp[0], bp ? bp[0] : p.bg, has_bg, hc0);
+		double qu = input_c<BG>(at(fr.data, rb - W), bp ? at(bp, rb - W) : p.bg, has_bg, hu) * 0.25;
+		double cc = input_c<BG>(at(fr.data, rb), bp ? at(bp, rb) : p.bg, has_bg, hc0);
 		double qc = cc * 0.25;
 #pragma unroll
 		for (int r = 0; r < PF && r < NR; ++r)
@@ -1555,7 +1562,7 @@ significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0
 			bool hd = false;
 			const double cd = input_c<BG>(dn[r], bn[r], has_bg, hd);
 			/* lanes 0 and 63 keep their own centre: their S is not read */
-			const double ql = wave_shift_right(qc, qc), qr = wave_shift_left(qc, qc);
+			const double ql = wave_shift_right(qc), qr = wave_shift_left(qc);
 			const double qd = cd * 0.25;
 			const int gy = ya + r;
 			/* non-temporal: C is read again only by later kernels, so it
@@ -1563,8 +1570,9 @@ significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0
 			 * from L2 */
 			const bool own = own_lane && gy >= own0 && gy < own1;
 			if (own)
-				__builtin_nontemporal_store(cc, &p.clean[fr.off + o + r * W]);
-			own_row<BG>(p, fr, own, o + r * W, cc, false, acc);
+				__builtin_nontemporal_store(cc,
+					(double *)((char *)(p.clean + fr.off + rb + r * W) + lane8));
+			own_row<BG>(p, fr, own, rb + r * W + lane, cc, false, acc);
 			emit_row(sh, slot, lane, t_min, cols,
 				laplacian_q<EXACT, TAME>(qc, qu, qd, ql, qr, false) / (2 * nz[r]));
 			any_bad |= s_lane && invalid_noise(nz[r]);
@@ -1643,7 +1651,9 @@ template <bool MODEL, bool EXACT, bool BG, bool BGMAP>
 __device__ __forceinline__ void
 significance_pass(SigShared &sh, ModelRegion *region, const Params &p, int iter)
 {
-	const int wave = threadIdx.x >> 6;
+	/* uniform, so that a wave's rows and row addresses live in scalar
+	 * registers */
+	const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 	const int mode = p.mode;
 	const bool wrap = mode == CUSMIC_WRAP;
 	const double t_cr = p.threshold, t_min = fmin(p.threshold, p.neighbor);
