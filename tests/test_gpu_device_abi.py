@@ -188,3 +188,52 @@ def test_noise_model_closed_form(cuda):
                                            frame_readnoise=[rn])
         np.testing.assert_array_equal(wm[0], want)
         assert_same_pixels(outs[0].cpu().numpy(), wo[0])
+
+
+@pytest.mark.parametrize("shape", [(2900, 3001), (1536, 1365)])
+def test_host_entry_point_staged_equals_device(cuda, shape):
+    """remove_cosmics on host arrays large enough to go through the pinned
+    staging buffers (planes of several 32 MiB chunks, the last one partial;
+    a mask plane of more than 4 MiB) equals remove_cosmics_device on the same
+    inputs, bit for bit; a bad map value is still reported before any
+    output is written."""
+    from cusmic_amd import _lib, scenes
+
+    say("host entry point, staged copies", shape)
+    h, w = shape
+    data, signal = scenes.frame(shape, 31, "cpu", n_stars=60, n_tracks=400, max_len=8)
+    d = np.ascontiguousarray(data.numpy())
+    d[::97, ::89] = np.nan
+    err = np.ascontiguousarray(scenes.error_map(signal, 2.0, 5.0).numpy())
+    bg = np.ascontiguousarray(np.linspace(-30.0, 30.0, d.size).reshape(shape))
+    m = np.ascontiguousarray((np.arange(d.size).reshape(shape) % 211 == 0).astype(np.uint8))
+    o = _lib.default_options()
+    o.border = _lib.WRAP
+    img = _lib.CusmicImage(d.ctypes.data, err.ctypes.data, None, None, bg.ctypes.data, m.ctypes.data, w, h)
+    out, flags = np.full(d.shape, 7.0), np.full(d.shape, 7, np.uint8)
+    msg = ctypes.create_string_buffer(256)
+    st = _lib.lib().remove_cosmics(ctypes.byref(img), ctypes.byref(o), out.ctypes.data, flags.ctypes.data, msg, 256)
+    assert st == _lib.OK, msg.value
+
+    dev = torch.device("cuda", 0)
+    td, te, tb, tm = (torch.from_numpy(a).to(dev) for a in (d, err, bg, m))
+    dimg = _lib.CusmicImage(td.data_ptr(), te.data_ptr(), None, None, tb.data_ptr(), tm.data_ptr(), w, h)
+    clean = torch.empty(d.size, dtype=torch.float64, device=dev)
+    mask = torch.empty(d.size, dtype=torch.uint8, device=dev)
+    torch.cuda.synchronize()
+    st, text = single_call(dimg, o, clean, mask, 0)
+    assert st == _lib.OK, text
+    torch.cuda.synchronize()
+    assert_same_pixels(out.ravel(), clean.cpu().numpy())
+    np.testing.assert_array_equal(flags.ravel(), mask.cpu().numpy())
+    assert flags.sum() > 1000
+
+    say("host entry point, staged copies, bad map value", shape)
+    err2 = err.copy()
+    err2[-1, -1] = 0.0
+    img = _lib.CusmicImage(d.ctypes.data, err2.ctypes.data, None, None, bg.ctypes.data, m.ctypes.data, w, h)
+    out2, flags2 = np.full(d.shape, 7.0), np.full(d.shape, 7, np.uint8)
+    st = _lib.lib().remove_cosmics(ctypes.byref(img), ctypes.byref(o), out2.ctypes.data, flags2.ctypes.data,
+                                   msg, 256)
+    assert st == _lib.INVALID and msg.value == b"error must be finite and positive"
+    assert (out2 == 7.0).all() and (flags2 == 7).all()
