@@ -547,7 +547,7 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	size_t n_counts = nf * (sizeof(Counts) / 4);
 	size_t n_shard = nf * size_t(SHARDS * SHARD_FIELDS);
 	size_t n_todo = size_t(PASS_RING) * nf;
-	size_t n_queue = 1;
+	size_t n_queue = XCDS;
 	size_t n_nlist = NL_GLOBALS + size_t(PASS_RING) * PASS_FIELDS;
 	size_t words = n_counts + n_shard + n_todo + n_queue + n_nlist + 2 + 5 * size_t(nblocks) + size_t(ntiles);
 	/* the processed-pixel count (64-bit) closes the zeroed words */

@@ -92,7 +92,12 @@ constexpr int CHUNK = 8192;
  * kernel: many workgroups adding to one address serialise at the memory side
  * (hundreds of nanoseconds each on this device). */
 constexpr int SHARDS = 32;
+
 enum { SHARD_HOLES, SHARD_ALLOWED, SHARD_DONORS, SHARD_FRESH, SHARD_FIELDS };
+
+/* Accelerator dies (XCDs) of the device, each with its own L2; workgroups
+ * are dispatched to them in turn. */
+constexpr int XCDS = 8;
 
 /* Per-frame counters of one call, zeroed at its start. fresh_passes counts
  * the detection passes that flagged a new pixel: the frame ran fresh_passes
@@ -155,7 +160,7 @@ struct Params {
 	int *shard;                      /* [nf][SHARDS][SHARD_FIELDS] pending counts */
 	int *todo;                       /* [PASS_RING][nf] new pixels of the pass
 					  * in each ring slot */
-	unsigned int *queue;             /* run queue head of the first pass */
+	unsigned int *queue;             /* [XCDS] run queue heads of the first pass */
 	unsigned int *nlist;             /* NL_* globals, then [PASS_RING][PASS_FIELDS] */
 	unsigned int *changed_tag;       /* [nblocks] stamp of the last pass whose
 					  * preceding replacement changed a pixel of

@@ -1180,7 +1180,9 @@ laplacian_q(double c, double u, double d, double l, double r, bool narrow)
 	double ul = clip<TAME>((((-u - l) + c4) - c) - c);
 	double ur = clip<TAME>((((-u - c) + c4) - r) - c);
 	double dl = clip<TAME>((((-c - l) + c4) - c) - d);
-	double dr = clip<TAME>((((-c - c) + c4) - r) - d);
+	/* (-c - c) + 4c is 2c exactly for finite c (sign of a zero aside); an
+	 * infinite c makes it NaN, which only a background can bring */
+	double dr = clip<TAME>(((TAME ? c + c : (-c - c) + c4) - r) - d);
 	if (narrow)
 		return ((ul + ur) + dl) + dr;
 	return (ul + ur) + (dl + dr);
@@ -1287,6 +1289,51 @@ med_input(const Frame &fr, double bg, bool has_bg, int x, int y, int mode, bool 
 	return bx < 0 || by < 0 ? 0.0 : input_at<BG>(fr, bg, has_bg, (long long)by * fr.w + bx, hole);
 }
 
+/* frames[f] through scalar loads, for a uniform f: the table is written
+ * before the kernel that reads it starts, so the scalar cache holds no stale
+ * copy. */
+__device__ __forceinline__ Frame
+frame_scalar(const Frame *frames, int f)
+{
+	static_assert(sizeof(Frame) % 4 == 0, "the table is read in 32-bit words");
+	using cword = const __attribute__((address_space(4))) unsigned;
+	Frame fr;
+	unsigned *dst = reinterpret_cast<unsigned *>(&fr);
+	cword *src = (cword *)(frames + f);
+#pragma unroll
+	for (unsigned k = 0; k < sizeof(Frame) / 4; ++k)
+		dst[k] = src[k];
+	return fr;
+}
+
+/* Raw buffer access: a resource whose base is a uniform address, a lane's byte
+ * offset in a vector register and a row's byte offset in a scalar one, so that
+ * a row's loads and stores need no vector address arithmetic. The record
+ * count is the largest offset the hardware accepts; callers keep offsets far
+ * below it. Word 3 is the gfx9 raw-buffer format (32-bit data format, no
+ * swizzle, no index). */
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t
+buffer_at(const void *base)
+{
+	return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, 0x7fffffff, 0x00020000);
+}
+
+__device__ __forceinline__ double
+buffer_load_f64(__amdgpu_buffer_rsrc_t r, unsigned lane_off, unsigned row_off)
+{
+	return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, lane_off, row_off, 0));
+}
+
+/* Non-temporal store (cache policy bit 1: nt). */
+__device__ __forceinline__ void
+buffer_store_f64_nt(double v, __amdgpu_buffer_rsrc_t r, unsigned lane_off, unsigned row_off)
+{
+	__builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, lane_off, row_off, 2);
+}
+
+
 /* Lane i takes lane i - 1's value (wave_shr:1); lane 0 gets 0 (bound_ctrl,
  * so no register has to hold an old value). */
 __device__ __forceinline__ double
@@ -1310,7 +1357,6 @@ constexpr int SEED_BUF = 512; /* seeds buffered per workgroup between flushes */
 
 struct SigShared {
 	double s[RING][SW];              /* S, row r of a run in slot r % RING */
-	unsigned long long screen[RING]; /* per row: tile columns with S > min(t, t_n) */
 	int step_bits[2][BLOCK / 64];    /* per step parity and wave: bit 0 an invalid
 					  * noise value among its rows, bit 1 a hole
 					  * among the samples of C they read */
@@ -1359,15 +1405,12 @@ flush_seeds(SigShared &sh, unsigned long long *list, unsigned int *count)
 		sh.nseed = 0;
 }
 
-/* Store one row of S (this lane's slot) and its screen. */
+/* Store one row of S (this lane's slot). The screen of a row is taken from
+ * the ring after the step's barrier. */
 __device__ __forceinline__ void
-emit_row(SigShared &sh, int &slot, int lane, double t_min, unsigned long long cols, double v)
+emit_row(SigShared &sh, int &slot, int lane, double v)
 {
 	sh.s[slot][lane] = v;
-	const unsigned long long m = __ballot(v > t_min) & cols;
-	/* every lane stores the same word: without a branch the loads of later
-	 * rows can be issued ahead of this one */
-	sh.screen[slot] = m;
 	slot = slot + 1 == RING ? 0 : slot + 1;
 }
 
@@ -1474,10 +1517,21 @@ model_row_region(SigShared &sh, int &slot, int lane, double t_min, unsigned long
 	own_row<BG>(p, fr, own, (long long)gy * w + gx, cc, true, acc);
 	/* lanes 0 and 63 carry C only: their S is never read */
 	const bool s_lane = lane >= 1 && lane <= SW - 2;
-	emit_row(sh, slot, lane, t_min, cols,
+	emit_row(sh, slot, lane,
 		laplacian_q<EXACT, !BG>(cc * 0.25, u * 0.25, d * 0.25, l * 0.25, r * 0.25, w == 1) / (2 * nz));
 	any_bad |= s_lane && (in || mode == CUSMIC_WRAP) && invalid_noise(nz);
 	any_hole |= hm;
+}
+
+/* Whether a step of NR rows from frame row ya takes the in-frame row loop:
+ * its columns, rows and their one-pixel ring lie inside the frame, and the
+ * rows' byte offsets fit the buffer offsets. */
+template <int NR>
+__device__ __forceinline__ bool
+inner_rows(int X0, int ya, int w, int h)
+{
+	return X0 - HS - 1 >= 0 && X0 - HS - 1 + SW <= w && ya - 1 >= 0 && ya + NR < h &&
+		(long long)(NR + 2) * w * 8 < (1ll << 31);
 }
 
 /* One wave: S of run rows [r0, r0 + NR) into the ring (run row r is frame row
@@ -1518,41 +1572,54 @@ significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0
 	const int gx = X0 - HS - 1 + lane, ya = yb + r0;
 	int slot = r0 % RING;
 	bool any_bad = false, any_hole = false;
-	/* the rows and their one-pixel ring lie inside the frame */
-	const bool inner = X0 - HS - 1 >= 0 && X0 - HS - 1 + SW <= w && ya - 1 >= 0 && ya + nr < h;
+	const bool inner = inner_rows<NR>(X0, ya, w, h);
 	if ((!MODEL || fr.error) && inner) {
-		/* global-space pointers: flat loads would also count against the
-		 * LDS wait counter */
-		using gchar = const __attribute__((address_space(1))) char;
-		using gdouble = const __attribute__((address_space(1))) double;
-		/* addresses: a uniform row base plus the lane's byte offset, so that
-		 * the loads and stores take a scalar base */
-		const long long rb = (long long)ya * w + (X0 - HS - 1), W = w;
-		const unsigned lane8 = unsigned(lane) * 8u;
-		auto at = [lane8](const double *base, long long idx) -> double {
-			return *(gdouble *)((gchar *)(base + idx) + lane8);
-		};
+		/* buffer resources at the wave's first column in row -1 (data,
+		 * background) and row 0 (noise, clean): row k of a plane is a scalar
+		 * offset, the lane a fixed vector one */
+		const long long rb = (long long)ya * w + (X0 - HS - 1);
+		const unsigned W8 = unsigned(w) * 8u, lane8 = unsigned(lane) * 8u;
 		const double *bp = BGMAP && fr.background ? fr.background : nullptr;
+		const __amdgpu_buffer_rsrc_t rd = buffer_at(fr.data + (rb - w));
+		const __amdgpu_buffer_rsrc_t re = buffer_at(fr.error + rb);
+		const __amdgpu_buffer_rsrc_t rc = buffer_at(p.clean + fr.off + rb);
+		const __amdgpu_buffer_rsrc_t rg = buffer_at(bp ? bp + (rb - w) : fr.data);
 		/* the samples of row r: the pixel below and the noise (and the
 		 * background below), fetched PF rows ahead */
 		constexpr int PF = BGMAP ? 4 : 5;
+		static_assert(PF <= NR, "the first rows are loaded before the loop");
 		double dn[NR], nz[NR], bn[NR];
 #define FETCH_ROW(k)                                                                       \
 	do {                                                                               \
-		const long long ro = rb + (long long)(k) * W;                              \
-		dn[k] = at(fr.data, ro + W);                                               \
-		nz[k] = at(fr.error, ro);                                                  \
+		dn[k] = buffer_load_f64(rd, lane8, unsigned((k) + 2) * W8);                \
+		nz[k] = buffer_load_f64(re, lane8, unsigned(k) * W8);                      \
 		bn[k] = p.bg;                                                              \
 		if (BG && bp)                                                              \
-			bn[k] = at(bp, ro + W);                                            \
+			bn[k] = buffer_load_f64(rg, lane8, unsigned((k) + 2) * W8);        \
 	} while (0)
+		/* Without a background C is the input itself, also at a hole: a
+		 * hole's C is written again by the hole fill before anything reads
+		 * it, and the output takes the input there. So every sample of a
+		 * row is used as loaded, and a hole among them shows as a NaN in
+		 * the sum d * 0 over every sample this lane loaded (lanes 0 and 63:
+		 * also the rows above and below theirs, which only makes the flag
+		 * cautious). */
+		double hsum = 0.0;
 		bool hu = false, hc0 = false;
-		double qu = input_c<BG>(at(fr.data, rb - W), bp ? at(bp, rb - W) : p.bg, has_bg, hu) * 0.25;
-		double cc = input_c<BG>(at(fr.data, rb), bp ? at(bp, rb) : p.bg, has_bg, hc0);
-		double qc = cc * 0.25;
+		const double du = buffer_load_f64(rd, lane8, 0), d0 = buffer_load_f64(rd, lane8, W8);
 #pragma unroll
-		for (int r = 0; r < PF && r < NR; ++r)
+		for (int r = 0; r < PF; ++r)
 			FETCH_ROW(r);
+		double qu, cc;
+		if (TAME) {
+			hsum = __builtin_fma(du, 0.0, __builtin_fma(d0, 0.0, hsum));
+			qu = du * 0.25;
+			cc = d0;
+		} else {
+			qu = input_c<BG>(du, bp ? buffer_load_f64(rg, lane8, 0) : p.bg, has_bg, hu) * 0.25;
+			cc = input_c<BG>(d0, bp ? buffer_load_f64(rg, lane8, W8) : p.bg, has_bg, hc0);
+		}
+		double qc = cc * 0.25;
 		/* fully unrolled and branch-free (constant NR, convergent ballots):
 		 * the loads run PF rows ahead */
 #pragma unroll
@@ -1560,30 +1627,33 @@ significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0
 			if (r + PF < NR)
 				FETCH_ROW(r + PF);
 			bool hd = false;
-			const double cd = input_c<BG>(dn[r], bn[r], has_bg, hd);
+			double cd;
+			if (TAME) {
+				cd = dn[r];
+				hsum = __builtin_fma(cd, 0.0, hsum);
+			} else {
+				cd = input_c<BG>(dn[r], bn[r], has_bg, hd);
+			}
 			/* lanes 0 and 63 keep their own centre: their S is not read */
 			const double ql = wave_shift_right(qc), qr = wave_shift_left(qc);
 			const double qd = cd * 0.25;
-			const int gy = ya + r;
-			/* non-temporal: C is read again only by later kernels, so it
-			 * should not displace the input lines that neighbouring tiles read
-			 * from L2 */
-			const bool own = own_lane && gy >= own0 && gy < own1;
-			if (own)
-				__builtin_nontemporal_store(cc,
-					(double *)((char *)(p.clean + fr.off + rb + r * W) + lane8));
-			own_row<BG>(p, fr, own, rb + r * W + lane, cc, false, acc);
-			emit_row(sh, slot, lane, t_min, cols,
-				laplacian_q<EXACT, TAME>(qc, qu, qd, ql, qr, false) / (2 * nz[r]));
-			any_bad |= s_lane && invalid_noise(nz[r]);
+			/* C at the run's own pixels. Non-temporal: C is read again only by
+			 * later kernels, so it should not displace the input lines that
+			 * neighbouring tiles read from L2. */
+			if (own_lane && ya + r >= own0 && ya + r < own1)
+				buffer_store_f64_nt(cc, rc, lane8, unsigned(r) * W8);
+			if (BG) {
+				const int gy = ya + r;
+				own_row<BG>(p, fr, own_lane && gy >= own0 && gy < own1, rb + r * w + lane, cc, false, acc);
+			}
+			emit_row(sh, slot, lane, laplacian_q<EXACT, TAME>(qc, qu, qd, ql, qr, false) / (2 * nz[r]));
+			any_bad |= s_lane && nz[r] <= 0;
 			any_hole |= hd;
 			qu = qc;
 			qc = qd;
 			cc = cd;
 		}
-		/* every sample of C this lane loaded (lanes 0 and 63: also the rows
-		 * above and below theirs, which only makes the flag cautious) */
-		any_hole |= hu || hc0;
+		any_hole |= hu || hc0 || hsum != 0.0;
 #undef FETCH_ROW
 	} else if (!MODEL || fr.error) {
 		bool hu = false, hc = false;
@@ -1599,8 +1669,7 @@ significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0
 			const double nz = fr.error[map_index(gx, gy, w, h, mode)];
 			const bool in = gx >= 0 && gx < w && gy >= 0 && gy < h;
 			own_row<BG>(p, fr, own_lane && gy >= own0 && gy < own1, (long long)gy * w + gx, cc, true, acc);
-			emit_row(sh, slot, lane, t_min, cols,
-				laplacian_q<EXACT, TAME>(cc * 0.25, qu, cd * 0.25, ql, qr, narrow) / (2 * nz));
+			emit_row(sh, slot, lane, laplacian_q<EXACT, TAME>(cc * 0.25, qu, cd * 0.25, ql, qr, narrow) / (2 * nz));
 			any_bad |= s_lane && (in || wrap) && invalid_noise(nz);
 			any_hole |= hc || (s_lane && (hu || hd || hs));
 			qu = cc * 0.25;
@@ -1658,12 +1727,30 @@ significance_pass(SigShared &sh, ModelRegion *region, const Params &p, int iter)
 	const bool wrap = mode == CUSMIC_WRAP;
 	const double t_cr = p.threshold, t_min = fmin(p.threshold, p.neighbor);
 	const unsigned dtag = tag_stamp(iter);
-	const unsigned limit = (unsigned)p.nruns;
 	unsigned int *seeds = &p.nlist[nl_seed(p, iter)];
-	unsigned next = 0;
+	/* The runs are split into XCDS contiguous ranges, one queue each; a
+	 * workgroup claims from the queue of its XCD (workgroups are dispatched to
+	 * the XCDs in turn) and, once that is empty, from the others in turn.
+	 * Consecutive runs are neighbouring tile columns, so the runs that share
+	 * a halo column start together under one L2. Every queue ends empty, so
+	 * every workgroup exits. */
+	int tries = 0;
+	auto claim = [&]() -> int {
+		while (tries < XCDS) {
+			const unsigned x = (blockIdx.x + tries) % XCDS;
+			const unsigned lo = unsigned((long long)p.nruns * x / XCDS);
+			const unsigned hi = unsigned((long long)p.nruns * (x + 1) / XCDS);
+			const unsigned k = atomicAdd(&p.queue[x], 1u);
+			if (k < hi - lo)
+				return int(lo + k);
+			++tries;
+		}
+		return -1;
+	};
+	int next = -1;
 	unsigned long long work = 0; /* in-frame pixels processed, thread 0 */
 	if (!threadIdx.x) {
-		next = atomicAdd(p.queue, 1u);
+		next = claim();
 		sh.nseed = 0;
 		sh.own[0] = sh.own[1] = sh.own[2] = 0;
 	}
@@ -1671,18 +1758,20 @@ significance_pass(SigShared &sh, ModelRegion *region, const Params &p, int iter)
 	for (;;) {
 		lds_barrier();
 		if (!threadIdx.x) {
-			unsigned q = next;
-			sh.item = q < limit ? (int)q : -1;
-			if (q < limit)
-				next = atomicAdd(p.queue, 1u);
+			sh.item = next;
+			if (next >= 0)
+				next = claim();
 		}
 		lds_barrier();
 		const int item = __builtin_amdgcn_readfirstlane(sh.item);
 		if (item < 0)
 			break;
 		/* the run: tile column tx, tile rows [ty0, ty0 + K) of frame f */
-		const int f = p.run_frame[item];
-		const Frame fr = p.frames[f];
+		/* scalar loads (the tables are written before this kernel starts), so
+		 * the frame's pointers and sizes are uniform registers that buffer
+		 * resources can take */
+		const int f = *(const __attribute__((address_space(4))) int *)(p.run_frame + item);
+		const Frame fr = frame_scalar(p.frames, f);
 		const int lr = item - fr.run0, tx = lr % fr.tiles_x, ty0 = lr / fr.tiles_x * p.run_tiles;
 		const int K = min(p.run_tiles, fr.tiles_y - ty0);
 		const int w = fr.w, h = fr.h, X0 = tx * TILE_W, Y0 = ty0 * TILE_H, yb = Y0 - HS;
@@ -1785,7 +1874,9 @@ significance_pass(SigShared &sh, ModelRegion *region, const Params &p, int iter)
 					int row = wave * (TILE_H / 4) + j;
 					if (row >= nrows)
 						break;
-					unsigned long long m = unsafe ? cols : sh.screen[(r_lo + HS + row) % RING];
+					/* the row's screen: S > min(t, t_n) at the tile's columns */
+					unsigned long long m = unsafe ? cols
+						: __ballot(sh.s[(r_lo + HS + row) % RING][lane] > t_min) & cols;
 					lds_append(((m >> lane) & 1) != 0, (uint16_t)((row << 6) | lane), sh.alist, &sh.na);
 				}
 			}
@@ -1813,7 +1904,7 @@ significance_pass(SigShared &sh, ModelRegion *region, const Params &p, int iter)
 					if (!isfinite(d))
 						continue;
 					const bool screened = !skip && (unsafe ||
-						((sh.screen[(r_lo + HS + row) % RING] >> (col + HS + 1)) & 1));
+						sh.s[(r_lo + HS + row) % RING][col + HS + 1] > t_min);
 					if (!screened)
 						p.flags[fr.off + i] = uint8_t(EXCL);
 					++acc.masked;
