@@ -1468,14 +1468,33 @@ load_region(ModelRegion &R, const Params &p, const Frame &fr, int X0, int yc0, i
 	const int bx0 = border_index(X0 - 2 * HS - 1 + lane, w, mode);
 	const bool second = lane < CREG_COLS - 64;
 	const int bx1 = second ? border_index(X0 - 2 * HS - 1 + 64 + lane, w, mode) : -1;
-	for (int k = wave; k < nrows; k += BLOCK / 64) {
-		const int by = border_index(yc0 + k, h, mode);
+	/* every load of the wave's rows first, then C, the LDS stores and the
+	 * hole ballots: one memory round trip per step instead of one per row */
+	constexpr int MAXK = (CREG_ROWS + BLOCK / 64 - 1) / (BLOCK / 64);
+	double d0[MAXK], d1[MAXK], g0[MAXK], g1[MAXK];
+	bool in0[MAXK], in1[MAXK];
+	long long i0[MAXK], i1[MAXK];
+#pragma unroll
+	for (int j = 0; j < MAXK; ++j) {
+		const int k = wave + j * (BLOCK / 64);
+		const int by = k < nrows ? border_index(yc0 + k, h, mode) : -1;
+		in0[j] = by >= 0 && bx0 >= 0;
+		in1[j] = by >= 0 && bx1 >= 0;
+		i0[j] = (long long)by * w + bx0;
+		i1[j] = (long long)by * w + bx1;
+		d0[j] = in0[j] ? fr.data[i0[j]] : 0.0;
+		d1[j] = in1[j] ? fr.data[i1[j]] : 0.0;
+		g0[j] = BG && fr.background && in0[j] ? fr.background[i0[j]] : p.bg;
+		g1[j] = BG && fr.background && in1[j] ? fr.background[i1[j]] : p.bg;
+	}
+#pragma unroll
+	for (int j = 0; j < MAXK; ++j) {
+		const int k = wave + j * (BLOCK / 64);
+		if (k >= nrows)
+			break;
 		bool h0 = false, h1 = false;
-		double v0 = 0.0, v1 = 0.0;
-		if (by >= 0 && bx0 >= 0)
-			v0 = input_at<BG>(fr, p.bg, has_bg, (long long)by * w + bx0, h0);
-		if (by >= 0 && bx1 >= 0)
-			v1 = input_at<BG>(fr, p.bg, has_bg, (long long)by * w + bx1, h1);
+		const double v0 = in0[j] ? input_c<BG>(d0[j], g0[j], has_bg, h0) : 0.0;
+		const double v1 = in1[j] ? input_c<BG>(d1[j], g1[j], has_bg, h1) : 0.0;
 		R.c[k][lane] = v0;
 		if (second)
 			R.c[k][64 + lane] = v1;
