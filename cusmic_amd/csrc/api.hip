@@ -402,7 +402,7 @@ private:
 std::vector<Frame>
 frame_table(const cusmic_image *ims, size_t nf, const cusmic_options &o, const double *fg,
 	const double *frn, int *ntiles_out, int *nchunks_out = nullptr, int *nruns_out = nullptr,
-	int *nblocks_out = nullptr, int run_tiles = RUN_TILES)
+	int *nblocks_out = nullptr, int run_tiles = RUN_TILES, int tile_w = TILE_W)
 {
 	std::vector<Frame> frames(nf);
 	long long off = 0;
@@ -421,7 +421,7 @@ frame_table(const cusmic_image *ims, size_t nf, const cusmic_options &o, const d
 		fr.w = int(im.width);
 		fr.h = int(im.height);
 		fr.off = off;
-		fr.tiles_x = (fr.w + TILE_W - 1) / TILE_W;
+		fr.tiles_x = (fr.w + tile_w - 1) / tile_w;
 		fr.tiles_y = (fr.h + TILE_H - 1) / TILE_H;
 		fr.tile0 = tiles;
 		fr.chunk0 = chunks;
@@ -516,6 +516,8 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 				       : bg_any ? significance_maps<false, true, false> : significance_maps<false, false, false>)
 				: (bg_map ? significance_maps<true, true, true>
 				       : bg_any ? significance_maps<true, true, false> : significance_maps<true, false, false>);
+	/* the error-map pass takes wide tiles */
+	const int tile_w = model ? TILE_W : TILE_W2;
 	const int cus = device_cus();
 	int sig_per_cu = 0;
 	hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&sig_per_cu, sig, BLOCK, 0));
@@ -526,7 +528,7 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	for (; run_tiles > 1; run_tiles /= 2) {
 		long long runs = 0;
 		for (size_t f = 0; f < nf; ++f) {
-			const long long tx = ((long long)ims[f].width + TILE_W - 1) / TILE_W;
+			const long long tx = ((long long)ims[f].width + tile_w - 1) / tile_w;
 			const long long ty = ((long long)ims[f].height + TILE_H - 1) / TILE_H;
 			runs += tx * ((ty + run_tiles - 1) / run_tiles);
 		}
@@ -535,7 +537,7 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	}
 	int ntiles = 0, nchunks = 0, nruns = 0, nblocks = 0;
 	std::vector<Frame> frames =
-		frame_table(ims, nf, o, fg, frn, &ntiles, &nchunks, &nruns, &nblocks, run_tiles);
+		frame_table(ims, nf, o, fg, frn, &ntiles, &nchunks, &nruns, &nblocks, run_tiles, tile_w);
 	size_t total = 0;
 	for (const auto &fr : frames)
 		total += size_t(fr.w) * fr.h;
@@ -593,6 +595,7 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	p.mode = o.border;
 	p.laplacian = laplacian;
 	p.run_tiles = run_tiles;
+	p.tile_w = tile_w;
 	p.contrast = o.contrast;
 	p.threshold = o.cr_threshold;
 	p.neighbor = o.neighbor_threshold;

@@ -4,7 +4,7 @@
  * A call cleans a batch of frames of individual shapes. Every per-pixel plane
  * (clean, flags, mask, donors) is the frames' pixels concatenated in
  * descriptor order; frame f starts at Frame::off. Frames are cut into
- * TILE_W x TILE_H tiles numbered frame-major, row-major; Frame::tile0 is the
+ * tiles of Params::tile_w x TILE_H numbered frame-major, row-major; Frame::tile0 is the
  * first tile of frame f.
  */
 #pragma once
@@ -16,8 +16,11 @@ namespace cusmic {
 
 /* Detection tile: 58 x 32 outputs, so that the significance region with the
  * 2-pixel halo of the 5x5 median and the 1-pixel ring of the Laplacian is
- * exactly one wave (64 lanes) wide. */
+ * exactly one wave (64 lanes) wide. The first pass over error-map frames
+ * takes wide tiles of TILE_W2 x TILE_H, two columns per lane (120 + 2 x 4 =
+ * 128); Params::tile_w says which width a call's tile table uses. */
 constexpr int TILE_W = 58;
+constexpr int TILE_W2 = 120;
 constexpr int TILE_H = 32;
 constexpr int BLOCK = 256;
 
@@ -142,6 +145,7 @@ struct Params {
 	const Frame *frames;
 	int nf, ntiles, maxiter, mode, laplacian;
 	int run_tiles;                   /* tiles per first-pass run (<= RUN_TILES) */
+	int tile_w;                      /* tile width of the tile table: TILE_W or TILE_W2 */
 	int reach, reach_s;              /* REACH_MAPS or REACH, REACH_S_MAPS or REACH_S */
 	double contrast, threshold, neighbor, bg;  /* bg: scalar background or NaN */
 	double *clean;

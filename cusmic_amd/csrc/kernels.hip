@@ -1333,6 +1333,26 @@ buffer_store_f64_nt(double v, __amdgpu_buffer_rsrc_t r, unsigned lane_off, unsig
 	__builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, lane_off, row_off, 2);
 }
 
+/* Two neighbouring doubles in one 16-byte access. */
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ void
+buffer_load_f64x2(__amdgpu_buffer_rsrc_t r, unsigned lane_off, unsigned row_off, double &x, double &y)
+{
+	const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, lane_off, row_off, 0);
+	x = __hiloint2double(int(v.y), int(v.x));
+	y = __hiloint2double(int(v.w), int(v.z));
+}
+
+__device__ __forceinline__ void
+buffer_store_f64x2_nt(double x, double y, __amdgpu_buffer_rsrc_t r, unsigned lane_off, unsigned row_off)
+{
+	const u32x4 v = {unsigned(__double2loint(x)), unsigned(__double2hiint(x)), unsigned(__double2loint(y)),
+		unsigned(__double2hiint(y))};
+	__builtin_amdgcn_raw_buffer_store_b128(v, r, lane_off, row_off, 2);
+}
+
 
 /* Lane i takes lane i - 1's value (wave_shr:1); lane 0 gets 0 (bound_ctrl,
  * so no register has to hold an old value). */
@@ -1355,12 +1375,33 @@ wave_shift_left(double v)
 
 constexpr int SEED_BUF = 512; /* seeds buffered per workgroup between flushes */
 
+/* Tile geometry of a first pass. The noise-model pass takes TILE_W x TILE_H
+ * tiles and a 64-column window, one column per lane (the window's columns
+ * are frame columns X0 - OFF ..). The error-map pass takes TILE_W2 x TILE_H
+ * tiles and a 128-column window, two neighbouring columns per lane: its loads
+ * and stores move 16 bytes per lane, which that pass's memory access pattern
+ * is bound by. Window columns 0 and WIN - 1 carry C only; S is read at
+ * columns 1 .. WIN - 2. */
+template <bool WIDE>
+struct TileGeom {
+	static constexpr int TW = WIDE ? TILE_W2 : TILE_W; /* owned columns */
+	static constexpr int WIN = WIDE ? 128 : SW;        /* window columns */
+	static constexpr int OFF = WIDE ? 4 : HS + 1;      /* first owned window column */
+	static constexpr int CBITS = WIDE ? 7 : 6;         /* bits of a window column */
+	static_assert(OFF >= HS + 1 && OFF + TW + HS + 1 <= WIN, "the window holds the tile, S halo and ring");
+};
+static_assert(TILE_W2 % BLK == 0 && TILE_W2 % 2 == 0, "wide tiles cover whole blocks and 16-byte pairs");
+
+template <bool WIDE>
 struct SigShared {
-	double s[RING][SW];              /* S, row r of a run in slot r % RING */
+	using G = TileGeom<WIDE>;
+	/* S, row r of a run in slot r % RING (16-byte aligned: the wide row loop
+	 * writes two columns at once) */
+	__attribute__((aligned(16))) double s[RING][G::WIN];
 	int step_bits[2][BLOCK / 64];    /* per step parity and wave: bit 0 an invalid
 					  * noise value among its rows, bit 1 a hole
 					  * among the samples of C they read */
-	uint16_t alist[TILE_W * TILE_H]; /* screened pixels of the tile: row << 6 | lane */
+	uint16_t alist[G::TW * TILE_H];  /* screened pixels of the tile: row << CBITS | column */
 	unsigned long long seeds[SEED_BUF];
 	int na, nseed, item;
 	unsigned flush_base;
@@ -1390,8 +1431,9 @@ lds_barrier()
 }
 
 /* Move the buffered seeds of this workgroup to the global list. */
+template <class SH>
 __device__ void
-flush_seeds(SigShared &sh, unsigned long long *list, unsigned int *count)
+flush_seeds(SH &sh, unsigned long long *list, unsigned int *count)
 {
 	lds_barrier();
 	const int n = sh.nseed;
@@ -1407,8 +1449,9 @@ flush_seeds(SigShared &sh, unsigned long long *list, unsigned int *count)
 
 /* Store one row of S (this lane's slot). The screen of a row is taken from
  * the ring after the step's barrier. */
+template <class SH>
 __device__ __forceinline__ void
-emit_row(SigShared &sh, int &slot, int lane, double v)
+emit_row(SH &sh, int &slot, int lane, double v)
 {
 	sh.s[slot][lane] = v;
 	slot = slot + 1 == RING ? 0 : slot + 1;
@@ -1514,7 +1557,7 @@ load_region(ModelRegion &R, const Params &p, const Frame &fr, int X0, int yc0, i
  * median's mirror mode reflects instead, as in the region). */
 template <bool EXACT, bool BG>
 __device__ __forceinline__ void
-model_row_region(SigShared &sh, int &slot, int lane, double t_min, unsigned long long cols,
+model_row_region(SigShared<false> &sh, int &slot, int lane, double t_min, unsigned long long cols,
 	const Params &p, const Frame &fr, int gx, int gy, const double *rows, int k, double med, bool hm,
 	bool hc, bool own, OwnCounts &acc, bool &any_bad, bool &any_hole)
 {
@@ -1572,7 +1615,7 @@ inner_rows(int X0, int ya, int w, int h)
  * flight. */
 template <bool MODEL, bool EXACT, bool BG, bool BGMAP, int NR>
 __device__ __forceinline__ void
-significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0, int yb, int r0,
+significance_rows(SigShared<false> &sh, const Params &p, const Frame &fr, int f, int X0, int yb, int r0,
 	double t_min, unsigned long long cols, int own0, int own1, OwnCounts &acc,
 	const ModelRegion *R, int rS0, int *step_word)
 {
@@ -1729,6 +1772,198 @@ significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0
 	emit_step_bits(step_word, lane, any_bad, any_hole);
 }
 
+/* One wave: S of run rows [r0, r0 + NR) into the ring of a wide tile
+ * (TileGeom<true>). Lane l holds window columns 2 l and 2 l + 1 (frame columns
+ * X0 - 4 + 2 l and the next), so inside the frame its loads and stores move
+ * 16 bytes: the pixel below and the noise value of both columns, PF rows
+ * ahead, and C at the run's own pixels. The rows above and below a lane's
+ * samples are its own earlier and later rows; the left neighbour of column
+ * 2 l is lane l - 1's second column and the right one of 2 l + 1 lane
+ * l + 1's first (wave shifts). Window columns 0 and 127 carry C only.
+ *
+ * Near a frame edge the same loop runs on mapped samples: a window column or
+ * run row outside the frame reads the sample the Laplacian's border rule
+ * gives (src/filters.cuh:72-105: the edge pixel for mirror, reflect and
+ * nearest, the wrapped one for wrap, 0 for constant), with 8-byte loads where
+ * a lane's two columns are not neighbours in memory. So every in-frame S is
+ * exact; an out-of-frame slot is exact under wrap (the periodic extension)
+ * and otherwise rewritten by the caller, as are its noise values, which can
+ * only make the noise flag cautious. Frames whose byte size passes the
+ * buffer offsets take the per-sample loop of significance_rows instead.
+ * Otherwise as significance_rows: without a background C is the input itself
+ * and a hole shows as a NaN in the sum d * 0; the step's noise and hole flags
+ * are emitted at the end. */
+template <bool EXACT, bool BG, bool BGMAP, int NR>
+__device__ __forceinline__ void
+significance_rows_wide(SigShared<true> &sh, const Params &p, const Frame &fr, int X0, int yb, int r0,
+	int ncols, int own0, int own1, OwnCounts &acc, int *step_word)
+{
+	using G = TileGeom<true>;
+	constexpr bool TAME = !BG;
+	int lane = threadIdx.x & 63;
+	asm volatile("" : "+v"(lane));
+	const int mode = p.mode, w = fr.w, h = fr.h;
+	const bool wrap = mode == CUSMIC_WRAP, constant = mode == CUSMIC_CONSTANT;
+	const bool has_bg = BG && (fr.background || !isnan(p.bg));
+	const int c0 = 2 * lane, c1 = c0 + 1;  /* window columns */
+	const int x0 = X0 - G::OFF;           /* frame column of window column 0 */
+	const int gx0 = x0 + c0, gx1 = gx0 + 1;
+	const bool own_x = c0 >= G::OFF && c0 < G::OFF + ncols;
+	const bool own_y = c1 >= G::OFF && c1 < G::OFF + ncols;
+	/* columns whose S the medians may read */
+	const bool s_x = c0 >= 1, s_y = c1 <= G::WIN - 2;
+	const int ya = yb + r0;
+	bool any_bad = false, any_hole = false;
+	if ((long long)w * h * 8 < (1ll << 31)) {
+		/* the source column of each sample, and whether it is a zero of the
+		 * constant mode; a lane's two columns are neighbours in memory when
+		 * the window lies inside the frame */
+		auto src = [&](int x) { return wrap ? border_index(x, w, mode) : min(max(x, 0), w - 1); };
+		const bool zx = constant && (gx0 < 0 || gx0 >= w), zy = constant && (gx1 < 0 || gx1 >= w);
+		const bool pairs = x0 >= 0 && x0 + G::WIN <= w;
+		const unsigned vx = unsigned(pairs ? gx0 : src(gx0)) * 8u, vy = unsigned(pairs ? gx1 : src(gx1)) * 8u;
+		const unsigned W8 = unsigned(w) * 8u;
+		/* the byte offset of a run row's source row; zero: the row is outside
+		 * the frame under the constant mode */
+		auto row_off = [&](int y, bool &zero) -> unsigned {
+			zero = constant && (y < 0 || y >= h);
+			return unsigned(wrap ? border_index(y, h, mode) : min(max(y, 0), h - 1)) * W8;
+		};
+		const double *bp = BGMAP && fr.background ? fr.background : nullptr;
+		const __amdgpu_buffer_rsrc_t rd = buffer_at(fr.data);
+		const __amdgpu_buffer_rsrc_t re = buffer_at(fr.error);
+		const __amdgpu_buffer_rsrc_t rc = buffer_at(p.clean + fr.off);
+		const __amdgpu_buffer_rsrc_t rg = buffer_at(bp ? bp : fr.data);
+		auto load2 = [&](__amdgpu_buffer_rsrc_t r, unsigned roff, double &a, double &b) {
+			if (pairs) {
+				buffer_load_f64x2(r, vx, roff, a, b);
+			} else {
+				a = buffer_load_f64(r, vx, roff);
+				b = buffer_load_f64(r, vy, roff);
+			}
+		};
+		/* C from a loaded sample pair of a row (zero: constant-mode row
+		 * outside the frame) */
+		bool hh = false;
+		double hsum = 0.0;
+		auto make_c = [&](double d0, double d1, double b0, double b1, bool zero, double &o0, double &o1) {
+			if (TAME) {
+				hsum = __builtin_fma(d0, 0.0, __builtin_fma(d1, 0.0, hsum));
+				o0 = d0;
+				o1 = d1;
+			} else {
+				o0 = input_c<BG>(d0, b0, has_bg, hh);
+				o1 = input_c<BG>(d1, b1, has_bg, hh);
+			}
+			if (constant) {
+				o0 = zero || zx ? 0.0 : o0;
+				o1 = zero || zy ? 0.0 : o1;
+			}
+		};
+		constexpr int PF = BGMAP ? 3 : 4;
+		static_assert(PF <= NR, "the first rows are loaded before the loop");
+		double dx[NR], dy[NR], nx[NR], ny[NR], bx[NR], by[NR];
+		bool zr[NR];
+#define FETCH_ROW(k)                                                                       \
+	do {                                                                               \
+		bool zn;                                                                   \
+		const unsigned ro = row_off(ya + (k) + 1, zr[k]);                          \
+		load2(rd, ro, dx[k], dy[k]);                                               \
+		load2(re, row_off(ya + (k), zn), nx[k], ny[k]);                            \
+		bx[k] = by[k] = p.bg;                                                      \
+		if (BG && bp)                                                              \
+			load2(rg, ro, bx[k], by[k]);                                       \
+	} while (0)
+		bool zu, zc;
+		const unsigned ru = row_off(ya - 1, zu), rcur = row_off(ya, zc);
+		double ux, uy, cx, cy, gux = p.bg, guy = p.bg, gcx = p.bg, gcy = p.bg;
+		load2(rd, ru, ux, uy);
+		load2(rd, rcur, cx, cy);
+		if (BG && bp) {
+			load2(rg, ru, gux, guy);
+			load2(rg, rcur, gcx, gcy);
+		}
+#pragma unroll
+		for (int r = 0; r < PF; ++r)
+			FETCH_ROW(r);
+		make_c(ux, uy, gux, guy, zu, ux, uy);
+		make_c(cx, cy, gcx, gcy, zc, cx, cy);
+		double qux = ux * 0.25, quy = uy * 0.25, qcx = cx * 0.25, qcy = cy * 0.25;
+		int slot = r0 % RING;
+#pragma unroll
+		for (int r = 0; r < NR; ++r) {
+			if (r + PF < NR)
+				FETCH_ROW(r + PF);
+			double ex, ey;
+			make_c(dx[r], dy[r], bx[r], by[r], zr[r], ex, ey);
+			const double qdx = ex * 0.25, qdy = ey * 0.25;
+			const double qlx = wave_shift_right(qcy), qry = wave_shift_left(qcx);
+			const int gy = ya + r;
+			const bool own_r = gy >= own0 && gy < own1;
+			/* C at the run's own pixels (inside the frame), non-temporal as in
+			 * significance_rows */
+			const unsigned roff = unsigned(gy) * W8, lo = unsigned(gx0) * 8u;
+			if (own_r && own_x && own_y && pairs)
+				buffer_store_f64x2_nt(cx, cy, rc, lo, roff);
+			else {
+				if (own_r && own_x)
+					buffer_store_f64_nt(cx, rc, lo, roff);
+				if (own_r && own_y)
+					buffer_store_f64_nt(cy, rc, lo + 8u, roff);
+			}
+			if (BG) {
+				const long long i = (long long)gy * w + gx0;
+				own_row<BG>(p, fr, own_r && own_x, i, cx, false, acc);
+				own_row<BG>(p, fr, own_r && own_y, i + 1, cy, false, acc);
+			}
+			const bool narrow = w == 1;
+			const double sx = laplacian_q<EXACT, TAME>(qcx, qux, qdx, qlx, qcy, narrow) / (2 * nx[r]);
+			const double sy = laplacian_q<EXACT, TAME>(qcy, quy, qdy, qcx, qry, narrow) / (2 * ny[r]);
+			*reinterpret_cast<f64x2 *>(&sh.s[slot][c0]) = f64x2{sx, sy};
+			slot = slot + 1 == RING ? 0 : slot + 1;
+			any_bad |= (s_x && nx[r] <= 0) || (s_y && ny[r] <= 0);
+			qux = qcx;
+			quy = qcy;
+			qcx = qdx;
+			qcy = qdy;
+			cx = ex;
+			cy = ey;
+		}
+		any_hole |= hh || hsum != 0.0;
+#undef FETCH_ROW
+	} else {
+#pragma unroll
+		for (int e = 0; e < 2; ++e) {
+			const int gx = gx0 + e, col = c0 + e;
+			const bool own_c = e ? own_y : own_x, s_c = e ? s_y : s_x;
+			int slot = r0 % RING;
+			bool hu = false, hc = false;
+			double qu = lap_input<BG>(fr, p.bg, has_bg, gx, ya - 1, mode, hu) * 0.25;
+			double cc = lap_input<BG>(fr, p.bg, has_bg, gx, ya, mode, hc);
+#pragma unroll 1
+			for (int r = 0; r < NR; ++r) {
+				const int gy = ya + r;
+				bool hd = false, hs = false;
+				const double cd = lap_input<BG>(fr, p.bg, has_bg, gx, gy + 1, mode, hd);
+				const double ql = lap_input<BG>(fr, p.bg, has_bg, gx - 1, gy, mode, hs) * 0.25;
+				const double qr = lap_input<BG>(fr, p.bg, has_bg, gx + 1, gy, mode, hs) * 0.25;
+				const double nz = fr.error[map_index(gx, gy, w, h, mode)];
+				const bool in = gx >= 0 && gx < w && gy >= 0 && gy < h;
+				own_row<BG>(p, fr, own_c && gy >= own0 && gy < own1, (long long)gy * w + gx, cc, true, acc);
+				sh.s[slot][col] = laplacian_q<EXACT, TAME>(cc * 0.25, qu, cd * 0.25, ql, qr, w == 1) / (2 * nz);
+				slot = slot + 1 == RING ? 0 : slot + 1;
+				any_bad |= s_c && (in || wrap) && invalid_noise(nz);
+				any_hole |= hc || (s_c && (hu || hd || hs));
+				qu = cc * 0.25;
+				cc = cd;
+				hu = hc;
+				hc = hd;
+			}
+		}
+	}
+	emit_step_bits(step_word, lane, any_bad, any_hole);
+}
+
 /* The first detection pass, which also does the prologue's work (C, flags,
  * counts, hole list) for the pixels of each run. A tile whose samples of C
  * hold a hole gets no S' here: its S depends on the hole fill, which runs
@@ -1737,8 +1972,11 @@ significance_rows(SigShared &sh, const Params &p, const Frame &fr, int f, int X0
  * screened pixels carry SCR. */
 template <bool MODEL, bool EXACT, bool BG, bool BGMAP>
 __device__ __forceinline__ void
-significance_pass(SigShared &sh, ModelRegion *region, const Params &p, int iter)
+significance_pass(SigShared<!MODEL> &sh, ModelRegion *region, const Params &p, int iter)
 {
+	/* the error-map pass takes wide tiles (TileGeom) */
+	constexpr bool WIDE = !MODEL;
+	using G = TileGeom<WIDE>;
 	/* uniform, so that a wave's rows and row addresses live in scalar
 	 * registers */
 	const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1793,9 +2031,10 @@ significance_pass(SigShared &sh, ModelRegion *region, const Params &p, int iter)
 		const Frame fr = frame_scalar(p.frames, f);
 		const int lr = item - fr.run0, tx = lr % fr.tiles_x, ty0 = lr / fr.tiles_x * p.run_tiles;
 		const int K = min(p.run_tiles, fr.tiles_y - ty0);
-		const int w = fr.w, h = fr.h, X0 = tx * TILE_W, Y0 = ty0 * TILE_H, yb = Y0 - HS;
-		const int ncols = min(TILE_W, w - X0), own1 = min(Y0 + TILE_H * K, h);
-		const unsigned long long cols = ((1ull << ncols) - 1) << (HS + 1);
+		const int w = fr.w, h = fr.h, X0 = tx * G::TW, Y0 = ty0 * TILE_H, yb = Y0 - HS;
+		const int ncols = min(G::TW, w - X0), own1 = min(Y0 + TILE_H * K, h);
+		/* the tile's columns in the narrow window (one lane each) */
+		const unsigned long long cols = WIDE ? 0 : ((1ull << ncols) - 1) << (HS + 1);
 		const bool has_bg = BG && (fr.background || !isnan(p.bg));
 		OwnCounts acc{0, 0, 0, 0, false};
 
@@ -1815,7 +2054,15 @@ significance_pass(SigShared &sh, ModelRegion *region, const Params &p, int iter)
 				load_region<BG>(*region, p, fr, X0, yb + rS0 - HS, s == 0 ? CREG_ROWS : TILE_H + 2 * HS);
 				lds_barrier();
 			}
-			if (s == 0) {
+			if constexpr (WIDE) {
+				if (s == 0)
+					significance_rows_wide<EXACT, BG, BGMAP, STEP_ROWS / 4>(sh, p, fr, X0, yb,
+						wave * (STEP_ROWS / 4), ncols, Y0, own1, acc, &sh.step_bits[s & 1][wave]);
+				else
+					significance_rows_wide<EXACT, BG, BGMAP, TILE_H / 4>(sh, p, fr, X0, yb,
+						TILE_H * s + 2 * HS + wave * (TILE_H / 4), ncols, Y0, own1, acc,
+						&sh.step_bits[s & 1][wave]);
+			} else if (s == 0) {
 				const int r0 = wave * (STEP_ROWS / 4);
 				significance_rows<MODEL, EXACT, BG, BGMAP, STEP_ROWS / 4>(sh, p, fr, f, X0, yb, r0, t_min,
 					cols, Y0, own1, acc, region, rS0, &sh.step_bits[s & 1][wave]);
@@ -1835,15 +2082,15 @@ significance_pass(SigShared &sh, ModelRegion *region, const Params &p, int iter)
 			/* 2. out-of-frame slots the medians read: the mapped in-frame slot
 			 * (median border mode), or 0 for CONSTANT */
 			const int r_lo = TILE_H * s, y_lo = yb + r_lo;
-			const bool edge = X0 - HS < 0 || X0 + TILE_W + HS > w || y_lo < 0 || y_lo + STEP_ROWS > h;
+			const bool edge = X0 - HS < 0 || X0 + G::TW + HS > w || y_lo < 0 || y_lo + STEP_ROWS > h;
 			if (edge && !wrap) {
-				for (int k = tid; k < SW * STEP_ROWS; k += BLOCK) {
-					int sx = k & (SW - 1), sr = k / SW, x = X0 - HS - 1 + sx, y = y_lo + sr;
+				for (int k = tid; k < G::WIN * STEP_ROWS; k += BLOCK) {
+					int sx = k & (G::WIN - 1), sr = k / G::WIN, x = X0 - G::OFF + sx, y = y_lo + sr;
 					if ((x >= 0 && x < w && y >= 0 && y < h) || x < -2 || x > w + 1 || y < -2 || y > h + 1)
 						continue;
 					double v = 0.0;
 					if (mode != CUSMIC_CONSTANT)
-						v = sh.s[(border_index(y, h, mode) - yb) % RING][border_index(x, w, mode) - (X0 - HS - 1)];
+						v = sh.s[(border_index(y, h, mode) - yb) % RING][border_index(x, w, mode) - (X0 - G::OFF)];
 					sh.s[(r_lo + sr) % RING][sx] = v;
 				}
 				lds_barrier();
@@ -1880,8 +2127,8 @@ significance_pass(SigShared &sh, ModelRegion *region, const Params &p, int iter)
 					p.tile_skip[fr.tile0 + (ty0 + s) * fr.tiles_x + tx] = 1;
 				/* the tile's holes (only a skipped tile has any): the fill's
 				 * list and count. The trip count is uniform per wave. */
-				for (int k = tid; k < TILE_W * TILE_H; k += BLOCK) {
-					const int row = k / TILE_W, col = k % TILE_W, i = (ty + row) * w + X0 + col;
+				for (int k = tid; k < G::TW * TILE_H; k += BLOCK) {
+					const int row = k / G::TW, col = k % G::TW, i = (ty + row) * w + X0 + col;
 					const bool hole = row < nrows && col < ncols && !isfinite(fr.data[i]);
 					if (hole)
 						p.flags[fr.off + i] = uint8_t(HOLE | EXCL);
@@ -1894,9 +2141,18 @@ significance_pass(SigShared &sh, ModelRegion *region, const Params &p, int iter)
 					if (row >= nrows)
 						break;
 					/* the row's screen: S > min(t, t_n) at the tile's columns */
-					unsigned long long m = unsafe ? cols
-						: __ballot(sh.s[(r_lo + HS + row) % RING][lane] > t_min) & cols;
-					lds_append(((m >> lane) & 1) != 0, (uint16_t)((row << 6) | lane), sh.alist, &sh.na);
+					const double *srow = sh.s[(r_lo + HS + row) % RING];
+					if constexpr (WIDE) {
+						const int c0 = 2 * lane, c1 = c0 + 1;
+						const bool p0 = c0 >= G::OFF && c0 < G::OFF + ncols && (unsafe || srow[c0] > t_min);
+						const bool p1 = c1 >= G::OFF && c1 < G::OFF + ncols && (unsafe || srow[c1] > t_min);
+						lds_append(p0, (uint16_t)((row << G::CBITS) | c0), sh.alist, &sh.na);
+						lds_append(p1, (uint16_t)((row << G::CBITS) | c1), sh.alist, &sh.na);
+					} else {
+						unsigned long long m = unsafe ? cols : __ballot(srow[lane] > t_min) & cols;
+						lds_append(((m >> lane) & 1) != 0, (uint16_t)((row << G::CBITS) | lane), sh.alist,
+							&sh.na);
+					}
 				}
 			}
 			/* EXCL at the tile's masked pixels (the flags plane starts zeroed;
@@ -1905,25 +2161,25 @@ significance_pass(SigShared &sh, ModelRegion *region, const Params &p, int iter)
 			if (fr.mask) {
 				/* every mask byte of this thread is loaded at once; the data
 				 * only at masked pixels */
-				constexpr int PER = (TILE_W * TILE_H + BLOCK - 1) / BLOCK;
+				constexpr int PER = (G::TW * TILE_H + BLOCK - 1) / BLOCK;
 				uint8_t mv[PER];
 #pragma unroll
 				for (int j = 0; j < PER; ++j) {
-					const int k = tid + j * BLOCK, row = k / TILE_W, col = k % TILE_W;
-					const bool in = k < TILE_W * TILE_H && row < nrows && col < ncols;
+					const int k = tid + j * BLOCK, row = k / G::TW, col = k % G::TW;
+					const bool in = k < G::TW * TILE_H && row < nrows && col < ncols;
 					mv[j] = in ? fr.mask[(ty + row) * w + X0 + col] : 0;
 				}
 #pragma unroll
 				for (int j = 0; j < PER; ++j) {
 					if (!mv[j])
 						continue;
-					const int k = tid + j * BLOCK, row = k / TILE_W, col = k % TILE_W;
+					const int k = tid + j * BLOCK, row = k / G::TW, col = k % G::TW;
 					const int i = (ty + row) * w + X0 + col;
 					const double d = fr.data[i];
 					if (!isfinite(d))
 						continue;
 					const bool screened = !skip && (unsafe ||
-						sh.s[(r_lo + HS + row) % RING][col + HS + 1] > t_min);
+						sh.s[(r_lo + HS + row) % RING][col + G::OFF] > t_min);
 					if (!screened)
 						p.flags[fr.off + i] = uint8_t(EXCL);
 					++acc.masked;
@@ -1947,8 +2203,8 @@ significance_pass(SigShared &sh, ModelRegion *region, const Params &p, int iter)
 				bool seed = false;
 				int i = 0;
 				if (idx < na) {
-					int pos = sh.alist[idx], row = pos >> 6, c = pos & 63;
-					i = (ty + row) * w + X0 - HS - 1 + c;
+					int pos = sh.alist[idx], row = pos >> G::CBITS, c = pos & ((1 << G::CBITS) - 1);
+					i = (ty + row) * w + X0 - G::OFF + c;
 					int rc = r_lo + HS + row;
 					double v[25];
 #pragma unroll
@@ -2232,7 +2488,7 @@ significance_blocks_pass(BlockShared &sh, const Params &p, int iter)
 			}
 			p.stamp[g] = uint16_t(screened ? stamp : 0u);
 			if (!iter && seed)
-				seed = p.tile_skip[fr.tile0 + (y / TILE_H) * fr.tiles_x + x / TILE_W] != 0;
+				seed = p.tile_skip[fr.tile0 + (y / TILE_H) * fr.tiles_x + x / p.tile_w] != 0;
 		}
 		lds_append(seed, list_entry(f, y * w + x), buf, nseed);
 		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -2273,16 +2529,16 @@ significance_blocks(Params p, int iter)
 		significance_blocks_pass<MODEL, EXACT, true>(sh, p, iter);
 }
 
-/* The first pass when every frame has an error map. It is bound by load
- * latency, so the register budget is capped at four waves per SIMD (128
- * VGPRs, no spills), one workgroup more per CU than the compiler would choose.
- * BG: some frame has a background; BGMAP: some frame has a background map. */
+/* The first pass when every frame has an error map, on wide tiles. Its LDS
+ * (the 128-column ring) allows three workgroups per CU, and the register
+ * budget is set to match. BG: some frame has a background; BGMAP: some frame
+ * has a background map. */
 template <bool EXACT, bool BG, bool BGMAP>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void
 significance_maps(Params p, int iter)
 {
 	static_assert(BG || !BGMAP, "a background map is a background");
-	__shared__ SigShared sh;
+	__shared__ SigShared<true> sh;
 	significance_pass<false, EXACT, BG, BGMAP>(sh, nullptr, p, iter);
 }
 
@@ -2292,7 +2548,7 @@ template <bool EXACT, bool BG>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2))) void
 significance_model(Params p, int iter)
 {
-	__shared__ SigShared sh;
+	__shared__ SigShared<false> sh;
 	__shared__ ModelRegion region;
 	significance_pass<true, EXACT, BG, BG>(sh, &region, p, iter);
 }
