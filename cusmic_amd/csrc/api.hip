@@ -6,10 +6,12 @@
  * (src/api.cu:155-192) and the ragged-batch/adjoint extensions. One call
  * enqueues the loop on the caller's stream in chunks of passes, reads the
  * device counters between chunks to stop once every frame has converged, and
- * synchronises at the end to read the errors and pass counts.
+ * returns once the device has posted the errors and pass counts after the
+ * last kernel.
  */
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <cmath>
 #include <cstdio>
@@ -198,6 +200,83 @@ struct device_block {
 	}
 	device_block(const device_block &) = delete;
 	device_block &operator=(const device_block &) = delete;
+};
+
+/* Pinned host memory for a call's small transfers: the frame table goes up
+ * from it, and the device posts the counters the host reads into it (word 0:
+ * a sequence number, words 64 on: the counters). Copies between the device
+ * and pageable memory are staged through the runtime's buffer, tens of
+ * microseconds each, and a stream synchronisation costs about as much again
+ * after the device has finished; polling a word that a kernel writes does
+ * not. Blocks are taken from a pool, one per call in flight, and returned for
+ * the next call; they are never freed (pinned memory goes back to the system
+ * at process end). */
+struct mailbox_block {
+	void *ptr;
+	size_t bytes;
+	unsigned seq;
+};
+std::mutex mailbox_lock;
+std::vector<mailbox_block> mailbox_pool;
+
+struct mailbox {
+	mailbox_block b{nullptr, 0, 0};
+	explicit mailbox(size_t need)
+	{
+		{
+			std::lock_guard<std::mutex> hold(mailbox_lock);
+			for (size_t k = 0; k < mailbox_pool.size(); ++k)
+				if (mailbox_pool[k].bytes >= need) {
+					b = mailbox_pool[k];
+					mailbox_pool.erase(mailbox_pool.begin() + long(k));
+					return;
+				}
+		}
+		b.bytes = std::max<size_t>(need, size_t(64) << 10);
+		hip_check(hipHostMalloc(&b.ptr, b.bytes, hipHostMallocDefault));
+		*static_cast<unsigned int *>(b.ptr) = 0;
+	}
+	~mailbox()
+	{
+		std::lock_guard<std::mutex> hold(mailbox_lock);
+		mailbox_pool.push_back(b);
+	}
+	unsigned int *words() const { return static_cast<unsigned int *>(b.ptr) + 64; }
+	/* Enqueue the post of n words at dev, then wait until they are here:
+	 * poll the sequence word, asking the stream every few microseconds
+	 * whether it still has work (a failed kernel ends the wait with its
+	 * error); after POLL_MS the wait becomes a stream synchronisation, which
+	 * gives the host's core back during long calls. Nothing is enqueued after
+	 * the post before it arrives, so both waits end at the same point. */
+	void
+	post_and_wait(const unsigned int *dev, size_t n, hipStream_t stream)
+	{
+		constexpr double POLL_MS = 20.0;
+		const unsigned seq = ++b.seq;
+		unsigned int *flag = static_cast<unsigned int *>(b.ptr);
+		hipLaunchKernelGGL(post_counters, dim3(1), dim3(256), 0, stream, dev, flag, unsigned(n), seq);
+		hip_check(hipGetLastError());
+		const auto start = std::chrono::steady_clock::now();
+		for (unsigned k = 1; __atomic_load_n(flag, __ATOMIC_ACQUIRE) != seq; ++k) {
+			if (k % 256 == 0) {
+				const hipError_t e = hipStreamQuery(stream);
+				const bool long_wait = std::chrono::duration<double, std::milli>(
+					std::chrono::steady_clock::now() - start).count() > POLL_MS;
+				if (e != hipErrorNotReady || long_wait) {
+					if (e != hipErrorNotReady)
+						hip_check(e);
+					else
+						hip_check(hipStreamSynchronize(stream));
+					if (__atomic_load_n(flag, __ATOMIC_ACQUIRE) != seq)
+						throw std::runtime_error("the device did not post its counters");
+					break;
+				}
+			}
+			__builtin_ia32_pause();
+		}
+	}
+	mailbox(const mailbox &) = delete;
+	mailbox &operator=(const mailbox &) = delete;
 };
 
 /* Device address of the static scratch array on the current device, looked
@@ -551,10 +630,11 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	size_t n_todo = size_t(PASS_RING) * nf;
 	size_t n_queue = XCDS;
 	size_t n_nlist = NL_GLOBALS + size_t(PASS_RING) * PASS_FIELDS;
-	size_t words = n_counts + n_shard + n_todo + n_queue + n_nlist + 2 + 5 * size_t(nblocks) + size_t(ntiles);
-	/* the processed-pixel count (64-bit) closes the zeroed words */
-	const size_t work_word = align_up(words, 2);
-	words = work_word + 2;
+	/* the words the host reads come first, in one run that one copy brings
+	 * back: status (2) | processed pixels (64-bit) | counts | todo ring |
+	 * pass counters */
+	const size_t n_read = 4 + n_counts + n_todo + n_nlist;
+	size_t words = n_read + n_shard + n_queue + 5 * size_t(nblocks) + size_t(ntiles);
 	size_t off_frames = 0;
 	size_t off_words = align_up(nf * sizeof(Frame), 256);
 	size_t off_dlist = align_up(off_words + words * 4, 256);
@@ -584,8 +664,20 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	}
 	device_block mem(static_base ? 0 : bytes, stream);
 	char *base = static_cast<char *>(static_base ? static_base : mem.ptr);
-	hip_check(hipMemcpyAsync(base + off_frames, frames.data(), nf * sizeof(Frame),
-		hipMemcpyHostToDevice, stream));
+	/* the pinned block: sequence word and the counters the host reads | the
+	 * frame table; a call that may not allocate copies through pageable
+	 * memory and synchronises instead */
+	const size_t box_table = align_up(256 + n_read * 4, 256);
+	std::unique_ptr<mailbox> box(static_base ? nullptr : new mailbox(box_table + nf * sizeof(Frame)));
+	std::vector<unsigned int> read_host(box ? 0 : n_read);
+	unsigned int *read_words = box ? box->words() : read_host.data();
+	const void *table = frames.data();
+	if (box) {
+		table = static_cast<char *>(box->b.ptr) + box_table;
+		std::memcpy(const_cast<void *>(table), frames.data(), nf * sizeof(Frame));
+	}
+	hip_check(hipMemcpyAsync(base + off_frames, table, nf * sizeof(Frame), hipMemcpyHostToDevice,
+		stream));
 
 	Params p{};
 	p.frames = reinterpret_cast<const Frame *>(base + off_frames);
@@ -604,13 +696,14 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	p.mask = mask;
 	p.donors = donors;
 	unsigned int *w32 = reinterpret_cast<unsigned int *>(base + off_words);
-	p.counts = reinterpret_cast<Counts *>(w32);
-	p.shard = reinterpret_cast<int *>(w32 + n_counts);
-	p.todo = reinterpret_cast<int *>(w32 + n_counts + n_shard);
-	p.queue = w32 + n_counts + n_shard + n_todo;
-	p.nlist = p.queue + n_queue;
-	p.status = reinterpret_cast<int *>(p.nlist + n_nlist);
-	p.block_pass = reinterpret_cast<unsigned int *>(p.status + 2);
+	p.status = reinterpret_cast<int *>(w32);
+	p.work = reinterpret_cast<unsigned long long *>(w32 + 2);
+	p.counts = reinterpret_cast<Counts *>(w32 + 4);
+	p.todo = reinterpret_cast<int *>(w32 + 4 + n_counts);
+	p.nlist = w32 + 4 + n_counts + n_todo;
+	p.shard = reinterpret_cast<int *>(w32 + n_read);
+	p.queue = w32 + n_read + n_shard;
+	p.block_pass = p.queue + n_queue;
 	p.changed_tag = p.block_pass + nblocks;
 	p.dirty_tag = p.changed_tag + nblocks;
 	p.changed_sides = p.dirty_tag + nblocks;
@@ -619,7 +712,6 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	p.changed_list = reinterpret_cast<unsigned long long *>(base + off_clist);
 	p.nblocks = nblocks;
 	p.dirty_list = reinterpret_cast<unsigned long long *>(base + off_dlist);
-	p.work = reinterpret_cast<unsigned long long *>(w32 + work_word);
 	p.chunk_frame = reinterpret_cast<int *>(base + off_cframe);
 	p.run_frame = reinterpret_cast<int *>(base + off_rframe);
 	p.nruns = nruns;
@@ -696,8 +788,23 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	 * no frame flagged a new pixel in the chunk's last pass, or a replacement
 	 * found no donor. maxiter <= CHUNK_PASSES (the default 4) is one chunk
 	 * and no synchronisation inside the loop. */
-	std::vector<unsigned int> nl(n_nlist);
-	std::vector<int> todo_ring(new_pixels ? n_todo : 0);
+	/* the host's view of the counters, as of the last read_counters */
+	const int *status_word = reinterpret_cast<const int *>(read_words);
+	const Counts *counts = reinterpret_cast<const Counts *>(read_words + 4);
+	const int *todo_ring = reinterpret_cast<const int *>(read_words + 4 + n_counts);
+	const unsigned int *nl = read_words + 4 + n_counts + n_todo;
+	auto read_counters = [&]() {
+		hip_check(hipGetLastError());
+		if (box) {
+			box->post_and_wait(w32, n_read, stream);
+			/* the events of the stage timings must have completed */
+			if (profile)
+				hip_check(hipStreamSynchronize(stream));
+		} else {
+			hip_check(hipMemcpyAsync(read_words, w32, n_read * 4, hipMemcpyDeviceToHost, stream));
+			hip_check(hipStreamSynchronize(stream));
+		}
+	};
 	double list_sums[4] = {0, 0, 0, 0}; /* seeds, candidates, grown, widened */
 	/* the lists and new pixels of passes [a, b), read from the ring */
 	auto add_lists = [&](int a, int b) {
@@ -711,12 +818,7 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 					new_pixels[k] += todo_ring[size_t(pass_slot(k)) * nf + f];
 		}
 	};
-	auto read_todo = [&]() {
-		if (new_pixels)
-			hip_check(hipMemcpyAsync(todo_ring.data(), p.todo, n_todo * sizeof(int),
-				hipMemcpyDeviceToHost, stream));
-	};
-	int passes_run = 0, listed = 0, status_word = 0;
+	int passes_run = 0, listed = 0;
 	while (passes_run < maxiter) {
 		const int a = passes_run, b = std::min(maxiter, a + CHUNK_PASSES);
 		if (a > 0) {
@@ -729,14 +831,10 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 		passes_run = b;
 		if (b == maxiter)
 			break;
-		hip_check(hipGetLastError());
-		hip_check(hipMemcpyAsync(nl.data(), p.nlist, n_nlist * 4, hipMemcpyDeviceToHost, stream));
-		hip_check(hipMemcpyAsync(&status_word, p.status, sizeof(int), hipMemcpyDeviceToHost, stream));
-		read_todo();
-		hip_check(hipStreamSynchronize(stream));
+		read_counters();
 		add_lists(listed, b);
 		listed = b;
-		if (!nl[nl_pass(b - 1, PF_ACTIVE)] || (status_word & ERR_NO_DONORS))
+		if (!nl[nl_pass(b - 1, PF_ACTIVE)] || (*status_word & ERR_NO_DONORS))
 			break;
 	}
 	{
@@ -748,19 +846,8 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 		hipLaunchKernelGGL(epilogue, dim3(grid), dim3(BLOCK), 0, stream, p, (long long)total, bg_maps);
 	}
 	tm.mark(CUSMIC_PROFILE_EPILOGUE_MS);
-	hip_check(hipGetLastError());
-
-	std::vector<Counts> counts(nf);
-	unsigned long long work = 0;
-	hip_check(hipMemcpyAsync(counts.data(), p.counts, nf * sizeof(Counts), hipMemcpyDeviceToHost,
-		stream));
-	hip_check(hipMemcpyAsync(&status_word, p.status, sizeof(int), hipMemcpyDeviceToHost, stream));
-	if (profile) {
-		hip_check(hipMemcpyAsync(&work, p.work, 8, hipMemcpyDeviceToHost, stream));
-		hip_check(hipMemcpyAsync(nl.data(), p.nlist, n_nlist * 4, hipMemcpyDeviceToHost, stream));
-	}
-	read_todo();
-	hip_check(hipStreamSynchronize(stream));
+	read_counters();
+	const unsigned long long work = read_words[2] | (unsigned long long)read_words[3] << 32;
 	if (new_pixels && !profile)
 		add_lists(listed, passes_run);
 	/* frame f flagged new pixels in fresh_passes passes, each followed by a
@@ -786,7 +873,7 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 		profile[CUSMIC_PROFILE_GROWN] = list_sums[2];
 		profile[CUSMIC_PROFILE_WIDENED] = list_sums[3];
 	}
-	if (status_word & ERR_NO_DONORS)
+	if (*status_word & ERR_NO_DONORS)
 		throw std::invalid_argument("no finite replacement donors");
 	if (passes)
 		for (size_t f = 0; f < nf; ++f)

@@ -190,6 +190,22 @@ fold_counts(Params p, int iter, int first)
 	fold_shards(p, iter, first, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
+/* Copy the n counter words the host reads into its pinned block (after the
+ * block's first 64 words), then write seq to the block's first word, which
+ * the host polls. Every lane's stores are made visible at system scope before
+ * the barrier, and the sequence word is released after it, so a host that
+ * sees seq sees the words. One workgroup of 256 lanes. */
+__global__ __launch_bounds__(256) void
+post_counters(const unsigned int *words, unsigned int *box, unsigned n, unsigned seq)
+{
+	for (unsigned i = threadIdx.x; i < n; i += 256)
+		box[64 + i] = words[i];
+	__threadfence_system();
+	__syncthreads();
+	if (!threadIdx.x)
+		__hip_atomic_store(box, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 /* Clear the ring slots of passes [lo, hi] before a chunk reuses them (at most
  * PASS_RING passes; a fill kernel, not a memset, so that the call can be
  * captured in a graph). */
