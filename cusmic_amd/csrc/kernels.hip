@@ -2325,6 +2325,40 @@ sprime_at(const Params &p, long long off, int block0, int blocks_x, int x, int y
 	return current && !(fw & HOLE) ? p.sprime[g] : -INFINITY;
 }
 
+/* S' (as sprime_at) at the 3 x 3 neighbours of (x, y) in row-major order,
+ * -inf outside the frame or for an invalid lane: every neighbour's flags,
+ * block stamp and pixel stamp are loaded before any is used, then S' where it
+ * is current, so a lane waits for two round trips instead of nine. */
+__device__ __forceinline__ void
+neighbour_sprime(const Params &p, long long off, int block0, int blocks_x, int x, int y, int w, int h,
+	bool valid, double sp[9])
+{
+	uint32_t fw[9];
+	unsigned last[9], st[9];
+	bool in[9];
+#pragma unroll
+	for (int k = 0; k < 9; ++k) {
+		const int qx = x + k % 3 - 1, qy = y + k / 3 - 1;
+		in[k] = valid && qx >= 0 && qx < w && qy >= 0 && qy < h;
+		fw[k] = 0;
+		last[k] = 0;
+		st[k] = 0;
+		if (in[k]) {
+			const long long g = off + (long long)qy * w + qx;
+			fw[k] = p.flags[g];
+			last[k] = p.block_pass[block0 + (qy / BLK) * blocks_x + qx / BLK];
+			st[k] = p.stamp[g];
+		}
+	}
+#pragma unroll
+	for (int k = 0; k < 9; ++k) {
+		const bool current = last[k] ? st[k] == last[k] : (fw[k] & SCR) != 0;
+		sp[k] = -INFINITY;
+		if (in[k] && current && !(fw[k] & HOLE))
+			sp[k] = p.sprime[off + (long long)(y + k / 3 - 1) * w + x + k % 3 - 1];
+	}
+}
+
 constexpr int BR = BLK + 2 * HS; /* edge of the S region of a block */
 constexpr int WAVE_SEEDS = 256;  /* seeds buffered per wave between flushes */
 static_assert(BLK * BLK == 64, "one lane per pixel of a block");
@@ -2637,15 +2671,35 @@ fine_structure(const Params &p, const Frame &fr, int x, int y)
 		}
 	};
 	double acc[2][22], a[9], b[9], c[9], centre = 0.0;
+	/* the edge launch (one wave per SIMD) loads the whole block at once,
+	 * the inner one row by row */
+	double all[INNER ? 1 : 9][9];
 	if (block) {
-		load_row(y - 4, a);
-		load_row(y - 3, b);
+		if constexpr (INNER) {
+			load_row(y - 4, a);
+			load_row(y - 3, b);
+		} else {
+#pragma unroll
+			for (int r = 0; r < 9; ++r)
+				load_row(y - 4 + r, all[r]);
+#pragma unroll
+			for (int i = 0; i < 9; ++i) {
+				a[i] = all[0][i];
+				b[i] = all[1][i];
+			}
+		}
 	}
 #pragma unroll
 	for (int j = 0; j < 7; ++j) {
 		double row[7];
 		if (block) {
-			load_row(y - 2 + j, c);
+			if constexpr (INNER) {
+				load_row(y - 2 + j, c);
+			} else {
+#pragma unroll
+				for (int i = 0; i < 9; ++i)
+					c[i] = all[j + 2][i];
+			}
 			/* 3x3 medians from sorted columns: med9 = med3(max of minima,
 			 * med3 of middles, min of maxima) */
 			double lo[9], md[9], hi[9];
@@ -2686,7 +2740,8 @@ fine_structure(const Params &p, const Frame &fr, int x, int y)
 			centre = row[3];
 		median49_feed(j, acc[(j + 1) & 1], row, acc[j & 1]);
 		/* keep the loads of later rows from being hoisted here */
-		__builtin_amdgcn_sched_barrier(0);
+		if (INNER)
+			__builtin_amdgcn_sched_barrier(0);
 	}
 	const double med7 = acc[0][0];
 	double nz;
@@ -2782,15 +2837,20 @@ grow_first(Params p, int iter)
 			y = i / w;
 		}
 		__syncthreads();
+		/* all neighbours' S' in one round trip, then their atomics */
+		double sp[9];
+		neighbour_sprime(p, off, block0, blocks_x, x, y, w, h, e < n, sp);
+		uint32_t old[9];
+#pragma unroll
 		for (int k = 0; k < 9; ++k) {
-			int qx = x + k % 3 - 1, qy = y + k / 3 - 1;
-			bool join = false;
-			int q = qy * w + qx;
-			if (e < n && qx >= 0 && qx < w && qy >= 0 && qy < h &&
-			    sprime_at(p, off, block0, blocks_x, qx, qy, w) > p.threshold)
-				join = !(flags_or(&p.flags[off + q], GROW) & GROW);
-			lds_append(join, list_entry(f, q), out.item, &out.n);
+			old[k] = GROW;
+			if (sp[k] > p.threshold)
+				old[k] = flags_or(&p.flags[off + (y + k / 3 - 1) * w + x + k % 3 - 1], GROW);
 		}
+#pragma unroll
+		for (int k = 0; k < 9; ++k)
+			lds_append(!(old[k] & GROW), list_entry(f, (y + k / 3 - 1) * w + x + k % 3 - 1), out.item,
+				&out.n);
 		block_list_flush(out, p.list_b, count);
 	}
 }
@@ -2824,22 +2884,30 @@ grow_second(Params p, int iter)
 			flags_clear(&p.flags[off + i], GROW);
 		}
 		__syncthreads();
+		/* all neighbours' S' in one round trip, then their atomics */
+		double sp[9];
+		neighbour_sprime(p, off, block0, blocks_x, x, y, w, h, e < n, sp);
+		uint32_t old[9];
+#pragma unroll
 		for (int k = 0; k < 9; ++k) {
-			int qx = x + k % 3 - 1, qy = y + k / 3 - 1;
-			bool fresh = false;
-			int q = qy * w + qx;
-			if (e < n && qx >= 0 && qx < w && qy >= 0 && qy < h &&
-			    sprime_at(p, off, block0, blocks_x, qx, qy, w) > p.neighbor) {
-				uint32_t old = flags_or(&p.flags[off + q], CRAY);
-				if (!(old & CRAY)) {
-					flags_or(&p.flags[off + q], NEWP);
-					fresh = true;
-					fresh_n += 1;
-					lost_n += !(old & EXCL) && isfinite(p.clean[off + q]);
-				}
-			}
-			lds_append(fresh, list_entry(f, q), out.item, &out.n);
+			old[k] = CRAY;
+			if (sp[k] > p.neighbor)
+				old[k] = flags_or(&p.flags[off + (y + k / 3 - 1) * w + x + k % 3 - 1], CRAY);
 		}
+		bool fresh[9];
+#pragma unroll
+		for (int k = 0; k < 9; ++k) {
+			const long long q = off + (y + k / 3 - 1) * w + x + k % 3 - 1;
+			fresh[k] = !(old[k] & CRAY);
+			if (fresh[k]) {
+				flags_or(&p.flags[q], NEWP);
+				fresh_n += 1;
+				lost_n += !(old[k] & EXCL) && isfinite(p.clean[q]);
+			}
+		}
+#pragma unroll
+		for (int k = 0; k < 9; ++k)
+			lds_append(fresh[k], list_entry(f, (y + k / 3 - 1) * w + x + k % 3 - 1), out.item, &out.n);
 		int *shard = p.shard + (blockIdx.x % SHARDS) * SHARD_FIELDS;
 		wave_frame_sum(fresh_n, f, shard + SHARD_FRESH, SHARDS * SHARD_FIELDS);
 		wave_frame_sum(-lost_n, f, shard + SHARD_DONORS, SHARDS * SHARD_FIELDS);
