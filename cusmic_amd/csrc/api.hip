@@ -761,7 +761,8 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 			    : (laplacian ? significance_blocks<false, false> : significance_blocks<false, true>);
 	int blocks_per_cu = 0;
 	hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, blocks, BLOCK, 0));
-	const unsigned blocks_grid = unsigned(cus * std::max(blocks_per_cu, 1));
+	/* a multiple of XCDS: the block pass splits its list between the XCDs */
+	const unsigned blocks_grid = unsigned(std::max(cus * std::max(blocks_per_cu, 1) / XCDS, 1) * XCDS);
 	auto enqueue_pass = [&](int it) {
 		/* pass 0: the blocks of tiles with a hole, after the fill; later
 		 * passes: the blocks near a pixel the last replacement changed */

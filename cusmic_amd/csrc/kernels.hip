@@ -489,7 +489,9 @@ __global__ __launch_bounds__(BLOCK) void
 expand_dirty(Params p, int iter)
 {
 	static_assert(REACH < BLK, "a change reaches the adjacent blocks only");
-	__shared__ BlockList<BLOCK * 9> out;
+	__shared__ BlockList<BLOCK * 25> out;
+	__shared__ int wave_total[BLOCK / 64];
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	const unsigned n = p.nlist[nl_changed(p, iter)], tag = tag_stamp(iter);
 	const bool wrap = p.mode == CUSMIC_WRAP;
 	const int r = wrap ? 2 : 1, side = 2 * r + 1;
@@ -523,33 +525,60 @@ expand_dirty(Params p, int iter)
 			sides = p.changed_sides[b];
 			p.changed_sides[b] = 0;
 		}
-		/* rounds of at most nine neighbours, one buffer flush each */
-		for (int k0 = 0; k0 < side * side; k0 += 9) {
-			__syncthreads();
-			for (int k = k0; k < k0 + 9 && k < side * side; ++k) {
-				const int dx = k % side - r, dy = k / side - r;
-				int qx = bx + dx, qy = by + dy;
-				bool fresh = false;
-				auto reaches = [&](unsigned sd) {
-					return (dx == 0 || (sd & (dx < 0 ? SIDE_L : SIDE_R))) &&
-					       (dy == 0 || (sd & (dy < 0 ? SIDE_U : SIDE_D)));
-				};
-				const bool near_s = wrap || reaches(sides >> SIDES_S);
-				const bool reached = near_s || reaches(sides & 15u);
-				if (wrap) {
-					qx = (qx % nbx + nbx) % nbx;
-					qy = (qy % nby + nby) % nby;
-				}
-				const int q = block0 + qy * nbx + qx;
-				unsigned int *tags = near_s ? p.dirty_tag : p.reseed_tag;
-				if (e < n && reached && qx >= 0 && qx < nbx && qy >= 0 && qy < nby)
-					fresh = __atomic_load_n(&tags[q], __ATOMIC_RELAXED) != tag &&
-						atomicExch(&tags[q], tag) != tag;
-				lds_append(fresh, list_entry(f, int(unsigned(q) | (near_s ? 0u : RESEED))), out.item,
-					&out.n);
+		/* Every reached neighbour's tag first; then the fresh ones go to the
+		 * buffer in lane order, each lane's neighbours together, so that the
+		 * blocks around one change sit side by side in the list and the block
+		 * pass takes them together (under one L2). */
+		auto neighbour = [&](int k, int &q, bool &near_s) {
+			const int dx = k % side - r, dy = k / side - r;
+			int qx = bx + dx, qy = by + dy;
+			auto reaches = [&](unsigned sd) {
+				return (dx == 0 || (sd & (dx < 0 ? SIDE_L : SIDE_R))) &&
+				       (dy == 0 || (sd & (dy < 0 ? SIDE_U : SIDE_D)));
+			};
+			near_s = wrap || reaches(sides >> SIDES_S);
+			const bool reached = near_s || reaches(sides & 15u);
+			if (wrap) {
+				qx = (qx % nbx + nbx) % nbx;
+				qy = (qy % nby + nby) % nby;
 			}
-			block_list_flush(out, p.dirty_list, count);
+			q = block0 + qy * nbx + qx;
+			return e < n && reached && qx >= 0 && qx < nbx && qy >= 0 && qy < nby;
+		};
+		unsigned fresh = 0;
+		for (int k = 0; k < side * side; ++k) {
+			int q;
+			bool near_s;
+			if (neighbour(k, q, near_s)) {
+				unsigned int *tags = near_s ? p.dirty_tag : p.reseed_tag;
+				if (__atomic_load_n(&tags[q], __ATOMIC_RELAXED) != tag && atomicExch(&tags[q], tag) != tag)
+					fresh |= 1u << k;
+			}
 		}
+		const int cnt = __popc(fresh);
+		int incl = cnt;
+		for (int d = 1; d < 64; d <<= 1) {
+			const int t = __shfl_up(incl, d, 64);
+			if (lane >= d)
+				incl += t;
+		}
+		if (lane == 63)
+			wave_total[wave] = incl;
+		__syncthreads();
+		int at = incl - cnt, total = 0;
+		for (int k = 0; k < BLOCK / 64; ++k) {
+			at += k < wave ? wave_total[k] : 0;
+			total += wave_total[k];
+		}
+		for (unsigned m = fresh; m; m &= m - 1) {
+			int q;
+			bool near_s;
+			neighbour(__ffs(m) - 1, q, near_s);
+			out.item[at++] = list_entry(f, int(unsigned(q) | (near_s ? 0u : RESEED)));
+		}
+		if (!threadIdx.x)
+			out.n = total;
+		block_list_flush(out, p.dirty_list, count);
 	}
 }
 
@@ -2419,13 +2448,17 @@ significance_blocks_pass(BlockShared &sh, const Params &p, int iter)
 	if (!threadIdx.x)
 		sh.work = 0;
 	__syncthreads();
-	const unsigned waves = gridDim.x * (BLOCK / 64);
-	unsigned e = blockIdx.x * (BLOCK / 64) + wave;
+	/* Each XCD takes a contiguous part of the list (workgroups go to the
+	 * XCDs in turn; the grid is a multiple of XCDS), so that neighbouring
+	 * blocks, which sit together in the list, share one L2. */
+	const unsigned xcd = blockIdx.x % XCDS, waves = gridDim.x / XCDS * (BLOCK / 64);
+	const unsigned hi = unsigned((unsigned long long)n * (xcd + 1) / XCDS);
+	unsigned e = unsigned((unsigned long long)n * xcd / XCDS) + blockIdx.x / XCDS * (BLOCK / 64) + wave;
 	/* the next block's list entry is loaded while this one is processed */
-	unsigned long long next = e < n ? p.dirty_list[e] : 0;
-	for (; e < n; e += waves) {
+	unsigned long long next = e < hi ? p.dirty_list[e] : 0;
+	for (; e < hi; e += waves) {
 		const unsigned long long entry = uniform_u64(next);
-		if (e + waves < n)
+		if (e + waves < hi)
 			next = p.dirty_list[e + waves];
 		const int f = (int)(entry >> 32), b = (int)((unsigned)entry & ~RESEED);
 		const Frame fr = p.frames[f];
