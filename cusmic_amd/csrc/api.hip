@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <memory>
 #include <condition_variable>
 #include <mutex>
@@ -221,7 +222,9 @@ std::vector<mailbox_block> mailbox_pool;
 
 struct mailbox {
 	mailbox_block b{nullptr, 0, 0};
-	explicit mailbox(size_t need)
+	hipStream_t stream;
+	const int live_exceptions;
+	mailbox(size_t need, hipStream_t s) : stream(s), live_exceptions(std::uncaught_exceptions())
 	{
 		{
 			std::lock_guard<std::mutex> hold(mailbox_lock);
@@ -238,6 +241,11 @@ struct mailbox {
 	}
 	~mailbox()
 	{
+		/* a call that ends by an exception may have left a copy from or to
+		 * the block on its stream: let it finish before another call takes
+		 * the block */
+		if (std::uncaught_exceptions() > live_exceptions)
+			(void)hipStreamSynchronize(stream);
 		std::lock_guard<std::mutex> hold(mailbox_lock);
 		mailbox_pool.push_back(b);
 	}
@@ -249,7 +257,7 @@ struct mailbox {
 	 * gives the host's core back during long calls. Nothing is enqueued after
 	 * the post before it arrives, so both waits end at the same point. */
 	void
-	post_and_wait(const unsigned int *dev, size_t n, hipStream_t stream)
+	post_and_wait(const unsigned int *dev, size_t n)
 	{
 		constexpr double POLL_MS = 20.0;
 		const unsigned seq = ++b.seq;
@@ -668,7 +676,7 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	 * frame table; a call that may not allocate copies through pageable
 	 * memory and synchronises instead */
 	const size_t box_table = align_up(256 + n_read * 4, 256);
-	std::unique_ptr<mailbox> box(static_base ? nullptr : new mailbox(box_table + nf * sizeof(Frame)));
+	std::unique_ptr<mailbox> box(static_base ? nullptr : new mailbox(box_table + nf * sizeof(Frame), stream));
 	std::vector<unsigned int> read_host(box ? 0 : n_read);
 	unsigned int *read_words = box ? box->words() : read_host.data();
 	const void *table = frames.data();
@@ -797,7 +805,7 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	auto read_counters = [&]() {
 		hip_check(hipGetLastError());
 		if (box) {
-			box->post_and_wait(w32, n_read, stream);
+			box->post_and_wait(w32, n_read);
 			/* the events of the stage timings must have completed */
 			if (profile)
 				hip_check(hipStreamSynchronize(stream));
