@@ -644,10 +644,11 @@ network_zero(const uint8_t *flags, const double *clean, int w, int x0, int x1, i
 }
 
 /* Store the replacement of pixel i; record the donors when asked (ties in
- * raster order within the final window). Returns true if the bits changed. */
+ * raster order within the final window), unless the caller found them
+ * (known: ia, ib). Returns true if the bits changed. */
 __device__ bool
 store_replacement(const Params &p, const Frame &fr, int i, int32_t *donors, int n, double lo,
-	double hi, int neg0, int x0, int x1, int y0, int y1)
+	double hi, int neg0, int x0, int x1, int y0, int y1, bool known = false, int ia = -1, int ib = -1)
 {
 	const double *clean = p.clean + fr.off;
 	const uint8_t *flags = p.flags + fr.off;
@@ -656,8 +657,10 @@ store_replacement(const Params &p, const Frame &fr, int i, int32_t *donors, int 
 		out = network_zero(flags, clean, fr.w, x0, x1, y0, y1, n);
 	if (donors) {
 		int a = (n - 1) >> 1, b = n >> 1;
-		int ia = rank_index(flags, clean, fr.w, x0, x1, y0, y1, rank_key(lo, a, neg0), a);
-		int ib = a == b ? ia : rank_index(flags, clean, fr.w, x0, x1, y0, y1, rank_key(hi, b, neg0), b);
+		if (!known) {
+			ia = rank_index(flags, clean, fr.w, x0, x1, y0, y1, rank_key(lo, a, neg0), a);
+			ib = a == b ? ia : rank_index(flags, clean, fr.w, x0, x1, y0, y1, rank_key(hi, b, neg0), b);
+		}
 		donors[2 * (fr.off + i)] = ia;
 		donors[2 * (fr.off + i) + 1] = ib;
 	}
@@ -708,6 +711,29 @@ gather_window(const Params &p, const Frame &fr, int x, int y, double (&v)[K], in
 	return n;
 }
 
+/* rank_index over the 5x5 window around (x, y) as gather_window holds it:
+ * its order is the window's raster order without the centre (never a donor),
+ * and a non-donor holds +inf. Donor values do not change during a pass, so
+ * this equals the scan of the window in memory. */
+__device__ __forceinline__ int
+rank_index_near(const double (&v)[24], int x, int y, int w, unsigned long long key, int r)
+{
+	int less = 0;
+#pragma unroll
+	for (int k = 0; k < 24; ++k)
+		less += v[k] != INFINITY && order_key(v[k]) < key;
+	int skip = r - less, idx = -1;
+#pragma unroll
+	for (int k = 0; k < 24; ++k) {
+		const int o = k < 12 ? k : k + 1;
+		const bool hit = v[k] != INFINITY && order_key(v[k]) == key;
+		if (hit && skip == 0)
+			idx = (y + o / 5 - 2) * w + x + o % 5 - 2;
+		skip -= hit;
+	}
+	return idx;
+}
+
 /* Replacement from the 5x5 window (src/replace.cuh:118-150 with r = 2).
  * Returns 1 if the pixel changed, 0 if not, -1 if the window has no donor. */
 __device__ int
@@ -727,8 +753,14 @@ replace_near(const Params &p, const Frame &fr, int i, int32_t *donors)
 		lo = j == a ? s[j] : lo;
 		hi = j == b ? s[j] : hi;
 	}
+	/* the donors from the window in registers */
+	int ia = -1, ib = -1;
+	if (donors) {
+		ia = rank_index_near(v, x, y, w, rank_key(lo, a, neg0), a);
+		ib = a == b ? ia : rank_index_near(v, x, y, w, rank_key(hi, b, neg0), b);
+	}
 	return store_replacement(p, fr, i, donors, n, lo, hi, neg0, max(0, x - 2), min(w - 1, x + 2),
-		max(0, y - 2), min(h - 1, y + 2));
+		max(0, y - 2), min(h - 1, y + 2), true, ia, ib);
 }
 
 
