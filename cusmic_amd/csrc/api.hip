@@ -185,19 +185,81 @@ scratch_pool()
 	return pools[dev];
 }
 
-/* Stream-ordered device memory from the scratch pool, released at scope exit. */
+/* A call's device scratch. Every call waits for its kernels before it
+ * returns, so at scope exit a block is idle and goes back to a per-device
+ * cache instead of being freed; the next call takes the smallest cached block
+ * that fits. Only when none fits are the device's smaller cached blocks freed
+ * and a new block allocated from the scratch pool. Returning a block is a
+ * list insertion: a stream-ordered free took 0.36 ms per call inside torch's
+ * autograd backward. The cache keeps at most a quarter of the device's memory
+ * (beyond that a returned block is freed). A call that ends by an exception
+ * first waits for its stream. */
+struct cached_block {
+	void *ptr;
+	size_t bytes;
+	int dev;
+};
+std::mutex block_lock;
+std::vector<cached_block> block_cache;
+size_t device_bytes[POOL_DEVICES]; /* total memory per device, read once */
+
 struct device_block {
 	void *ptr = nullptr;
+	size_t bytes = 0;
+	int dev = 0;
 	hipStream_t stream;
-	device_block(size_t bytes, hipStream_t s) : stream(s)
+	const int live_exceptions;
+	device_block(size_t need, hipStream_t s) : stream(s), live_exceptions(std::uncaught_exceptions())
 	{
-		if (bytes)
-			hip_check(hipMallocFromPoolAsync(&ptr, bytes, scratch_pool(), s));
+		if (!need)
+			return;
+		hip_check(hipGetDevice(&dev));
+		std::vector<void *> idle;
+		{
+			std::lock_guard<std::mutex> hold(block_lock);
+			long best = -1;
+			for (size_t k = 0; k < block_cache.size(); ++k)
+				if (block_cache[k].dev == dev && block_cache[k].bytes >= need &&
+					(best < 0 || block_cache[k].bytes < block_cache[size_t(best)].bytes))
+					best = long(k);
+			if (best >= 0) {
+				ptr = block_cache[size_t(best)].ptr;
+				bytes = block_cache[size_t(best)].bytes;
+				block_cache.erase(block_cache.begin() + best);
+				return;
+			}
+			for (size_t k = block_cache.size(); k-- > 0;)
+				if (block_cache[k].dev == dev) {
+					idle.push_back(block_cache[k].ptr);
+					block_cache.erase(block_cache.begin() + long(k));
+				}
+		}
+		for (void *q : idle)
+			hip_check(hipFreeAsync(q, s));
+		hip_check(hipMallocFromPoolAsync(&ptr, need, scratch_pool(), s));
+		bytes = need;
 	}
 	~device_block()
 	{
-		if (ptr)
+		if (!ptr)
+			return;
+		if (std::uncaught_exceptions() > live_exceptions)
+			(void)hipStreamSynchronize(stream);
+		std::unique_lock<std::mutex> hold(block_lock);
+		size_t cached = bytes;
+		for (const auto &c : block_cache)
+			cached += c.dev == dev ? c.bytes : 0;
+		if (dev >= 0 && dev < POOL_DEVICES && !device_bytes[dev]) {
+			size_t free_bytes = 0, total_bytes = 0;
+			if (hipMemGetInfo(&free_bytes, &total_bytes) == hipSuccess)
+				device_bytes[dev] = total_bytes;
+		}
+		if (dev < 0 || dev >= POOL_DEVICES || cached > device_bytes[dev] / 4) {
+			hold.unlock();
 			(void)hipFreeAsync(ptr, stream);
+			return;
+		}
+		block_cache.push_back({ptr, bytes, dev});
 	}
 	device_block(const device_block &) = delete;
 	device_block &operator=(const device_block &) = delete;
@@ -1066,7 +1128,12 @@ cusmic_adjoint_device(const cusmic_image *ims, size_t nf, const uint8_t *mask,
 		const size_t off_list = align_up(off_words + n_words * 4, 256);
 		device_block mem(off_list + ADJ_LIST * 8, s);
 		char *base = static_cast<char *>(mem.ptr);
-		hip_check(hipMemcpyAsync(base, frames.data(), nf * sizeof(Frame), hipMemcpyHostToDevice, s));
+		/* the table goes up from a pinned block, which the target count
+		 * comes back to */
+		mailbox box(512 + nf * sizeof(Frame), s);
+		void *table = static_cast<char *>(box.b.ptr) + 512; /* after the posted words */
+		std::memcpy(table, frames.data(), nf * sizeof(Frame));
+		hip_check(hipMemcpyAsync(base, table, nf * sizeof(Frame), hipMemcpyHostToDevice, s));
 		int *reach = reinterpret_cast<int *>(base + off_words);
 		unsigned int *count = reinterpret_cast<unsigned int *>(reach + nf);
 		auto *list = reinterpret_cast<unsigned long long *>(base + off_list);
@@ -1078,10 +1145,9 @@ cusmic_adjoint_device(const cusmic_image *ims, size_t nf, const uint8_t *mask,
 		const unsigned wide_grid = unsigned(device_cus()) * 2;
 		hipLaunchKernelGGL(adjoint_wide, dim3(wide_grid), dim3(BLOCK), 0, s, dframes, mask, donors,
 			grad_clean, grad_data, reach, count, list, ADJ_LIST);
-		unsigned n = 0;
-		hip_check(hipMemcpyAsync(&n, count, sizeof(n), hipMemcpyDeviceToHost, s));
 		hip_check(hipGetLastError());
-		hip_check(hipStreamSynchronize(s));
+		box.post_and_wait(count, 1);
+		const unsigned n = box.words()[0];
 		if (n > ADJ_LIST) {
 			/* more targets than the list held: list them all and redo */
 			device_block all(size_t(n) * 8 + 256, s);
