@@ -391,6 +391,29 @@ device_cus()
 /* Host copies split over a few persistent threads: pageable memory moves to
  * and from pinned staging at several times one thread's memcpy rate. The
  * caller's thread takes the first slice. */
+/* Copy bytes of doubles and tell whether one of them fails the map check of
+ * check_values (minimum 1: finite and positive; 0: finite and nonnegative;
+ * -1: finite). NO_CHECK copies any bytes unchecked. */
+constexpr int NO_CHECK = -2;
+
+bool
+copy_checked(void *dst, const void *src, size_t bytes, int minimum)
+{
+	if (minimum == NO_CHECK) {
+		std::memcpy(dst, src, bytes);
+		return false;
+	}
+	const double *s = static_cast<const double *>(src);
+	double *d = static_cast<double *>(dst);
+	bool bad = false;
+	for (size_t i = 0, n = bytes / sizeof(double); i < n; ++i) {
+		const double v = s[i];
+		d[i] = v;
+		bad |= !std::isfinite(v) || (minimum == 1 && v <= 0) || (minimum == 0 && v < 0);
+	}
+	return bad;
+}
+
 class copy_threads {
 public:
 	static constexpr unsigned MAX_THREADS = 8;
@@ -402,15 +425,16 @@ public:
 		return *pool;
 	}
 
-	void copy(void *dst, const void *src, size_t bytes)
+	/* Copy on the workers and this thread; with a check (copy_checked),
+	 * returns whether a value failed it. */
+	bool copy(void *dst, const void *src, size_t bytes, int minimum = NO_CHECK)
 	{
 		const size_t slices = std::min<size_t>(workers_.size() + 1, std::max<size_t>(1, bytes >> 20));
-		if (slices <= 1) {
-			std::memcpy(dst, src, bytes);
-			return;
-		}
+		if (slices <= 1)
+			return copy_checked(dst, src, bytes, minimum);
 		std::lock_guard<std::mutex> one_job(job_lock_);
-		const size_t part = (bytes + slices - 1) / slices;
+		/* slices of whole doubles */
+		const size_t part = ((bytes + slices - 1) / slices + 7) & ~size_t(7);
 		{
 			std::lock_guard<std::mutex> l(m_);
 			dst_ = static_cast<char *>(dst);
@@ -418,13 +442,16 @@ public:
 			bytes_ = bytes;
 			part_ = part;
 			slices_ = slices;
+			minimum_ = minimum;
+			bad_ = false;
 			pending_ = slices - 1;
 			++generation_;
 		}
 		go_.notify_all();
-		std::memcpy(dst, src, std::min(part, bytes));
+		const bool mine = copy_checked(dst, src, std::min(part, bytes), minimum);
 		std::unique_lock<std::mutex> l(m_);
 		done_.wait(l, [&] { return pending_ == 0; });
+		return mine || bad_;
 	}
 
 private:
@@ -443,6 +470,7 @@ private:
 			char *dst;
 			const char *src;
 			size_t lo, hi;
+			int minimum;
 			{
 				std::unique_lock<std::mutex> l(m_);
 				go_.wait(l, [&] { return generation_ != seen; });
@@ -451,11 +479,13 @@ private:
 					continue;
 				dst = dst_;
 				src = src_;
+				minimum = minimum_;
 				lo = std::min(bytes_, slice * part_);
 				hi = std::min(bytes_, lo + part_);
 			}
-			std::memcpy(dst + lo, src + lo, hi - lo);
+			const bool bad = copy_checked(dst + lo, src + lo, hi - lo, minimum);
 			std::lock_guard<std::mutex> l(m_);
+			bad_ = bad_ || bad;
 			if (--pending_ == 0)
 				done_.notify_one();
 		}
@@ -467,6 +497,8 @@ private:
 	char *dst_ = nullptr;
 	const char *src_ = nullptr;
 	size_t bytes_ = 0, part_ = 0, slices_ = 0, pending_ = 0;
+	int minimum_ = NO_CHECK;
+	bool bad_ = false;
 	unsigned generation_ = 0;
 };
 
@@ -488,11 +520,20 @@ public:
 
 	std::mutex lock;
 
-	void upload(void *dst, const void *src, size_t bytes, hipStream_t stream)
+	/* With a check (copy_checked), the values are checked as they are
+	 * copied, and the return value tells whether one failed; the whole plane
+	 * is uploaded either way. */
+	bool upload(void *dst, const void *src, size_t bytes, hipStream_t stream, int minimum = NO_CHECK)
 	{
+		bool bad = false;
 		if (bytes < DIRECT) {
+			if (minimum != NO_CHECK) {
+				const double *v = static_cast<const double *>(src);
+				for (size_t i = 0, n = bytes / sizeof(double); i < n; ++i)
+					bad |= !std::isfinite(v[i]) || (minimum == 1 && v[i] <= 0) || (minimum == 0 && v[i] < 0);
+			}
 			hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
-			return;
+			return bad;
 		}
 		ensure();
 		char *d = static_cast<char *>(dst);
@@ -500,10 +541,11 @@ public:
 		for (size_t off = 0, k = 0; off < bytes; off += CHUNK, ++k) {
 			const size_t len = std::min(CHUNK, bytes - off), b = k & 1;
 			hip_check(hipEventSynchronize(done_[b]));
-			copy_threads::get().copy(pin_[b], s + off, len);
+			bad |= copy_threads::get().copy(pin_[b], s + off, len, minimum);
 			hip_check(hipMemcpyAsync(d + off, pin_[b], len, hipMemcpyHostToDevice, stream));
 			hip_check(hipEventRecord(done_[b], stream));
 		}
+		return bad;
 	}
 
 	void download(void *dst, const void *src, size_t bytes, hipStream_t stream)
@@ -951,22 +993,35 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 			passes[f] = detect_passes(f);
 }
 
-/* Host path (src/api.cu:116-153): the maps' values are checked first, on the
- * host; maxiter = 0 stays on the CPU; otherwise the planes are uploaded and
- * the results copied straight into the caller's buffers once the call has
- * succeeded, so that a failed call leaves them untouched. Both directions go
- * through the pinned staging buffers. */
+/* Host path (src/api.cu:116-153): the maps' values are checked on the host,
+ * in the reference's order and with its messages, before any kernel runs;
+ * maxiter = 0 stays on the CPU. Otherwise the planes are uploaded through the
+ * pinned staging buffers, each map checked as it is copied into them (one
+ * read of it), and the results go straight into the caller's buffers once
+ * the call has succeeded, so that a failed call leaves them untouched. */
 void
 clean_host(const cusmic_image &im, const cusmic_options &o, size_t n, double *clean_out,
 	uint8_t *mask_out)
 {
-	if (im.error)
-		check_values(im.error, n, "error must be finite and positive", 1);
-	else {
-		check_values(im.gain, n, "gain must be finite and positive", 1);
-		check_values(im.readnoise, n, "read noise must be finite and nonnegative", 0);
-	}
-	check_values(im.background, n, "background must be finite", -1);
+	/* the maps' checks, in the reference's order (src/api.cu:107-114) */
+	struct map_check {
+		const double *values;
+		const char *message;
+		int minimum;
+	};
+	const map_check checks[4] = {
+		{im.error, "error must be finite and positive", 1},
+		{im.error ? nullptr : im.gain, "gain must be finite and positive", 1},
+		{im.error ? nullptr : im.readnoise, "read noise must be finite and nonnegative", 0},
+		{im.background, "background must be finite", -1},
+	};
+	/* without a device to upload to (or with nothing to upload) the maps are
+	 * checked first, as the reference checks them before any device work */
+	int devices = 0;
+	const bool device = hipGetDeviceCount(&devices) == hipSuccess && devices > 0;
+	if (!o.maxiter || !device)
+		for (const auto &c : checks)
+			check_values(c.values, n, c.message, c.minimum);
 	if (!o.maxiter) {
 		if (clean_out != im.data)
 			std::memmove(clean_out, im.data, n * sizeof(double));
@@ -992,20 +1047,24 @@ clean_host(const cusmic_image &im, const cusmic_options &o, size_t n, double *cl
 	char *base = static_cast<char *>(mem.ptr);
 	double *d = reinterpret_cast<double *>(base);
 	double *next = d;
-	auto plane = [&](const double *src) -> const double * {
+	/* the maps are checked as they are copied to the staging buffers (one
+	 * read of each); a failed check raises before any kernel runs, and the
+	 * caller's buffers are untouched */
+	auto plane = [&](const double *src, const map_check *check) -> const double * {
 		if (!src)
 			return nullptr;
 		double *dst = next;
 		next += n;
-		stage.upload(dst, src, n * sizeof(double), stream);
+		if (stage.upload(dst, src, n * sizeof(double), stream, check ? check->minimum : NO_CHECK))
+			throw std::invalid_argument(check->message);
 		return dst;
 	};
 	cusmic_image gpu{};
-	gpu.data = plane(im.data);
-	gpu.error = plane(im.error);
-	gpu.gain = plane(gmap);
-	gpu.readnoise = plane(rnmap);
-	gpu.background = plane(im.background);
+	gpu.data = plane(im.data, nullptr);
+	gpu.error = plane(im.error, &checks[0]);
+	gpu.gain = plane(gmap, &checks[1]);
+	gpu.readnoise = plane(rnmap, &checks[2]);
+	gpu.background = plane(im.background, &checks[3]);
 	double *clean = next;
 	next += n;
 	uint8_t *mask = reinterpret_cast<uint8_t *>(next);

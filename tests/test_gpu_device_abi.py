@@ -9,6 +9,7 @@ NULL stream, and the closed-form noise-model scenes on the GPU.
 """
 
 import ctypes
+import math
 
 import numpy as np
 import pytest
@@ -237,3 +238,56 @@ def test_host_entry_point_staged_equals_device(cuda, shape):
                                    msg, 256)
     assert st == _lib.INVALID and msg.value == b"error must be finite and positive"
     assert (out2 == 7.0).all() and (flags2 == 7).all()
+
+
+@pytest.mark.parametrize("shape", [(2900, 3001), (48, 64)])
+def test_host_entry_point_checks_maps_while_uploading(cuda, shape):
+    """With a device, remove_cosmics checks each map as it copies it to the
+    staging buffers (planes of several 32 MiB chunks here, or under 4 MiB,
+    copied directly). Every bad value is reported with the reference's
+    message (src/api.cu:107-114), in its order (error, gain, read noise,
+    background), before any kernel runs and with the outputs untouched."""
+    from cusmic_amd import _lib, scenes
+
+    h, w = shape
+    data, signal = scenes.frame(shape, 7, "cpu", n_stars=4, n_tracks=20, max_len=4)
+    d = np.ascontiguousarray(data.numpy())
+    err = np.ascontiguousarray(scenes.error_map(signal, 2.0, 5.0).numpy())
+    gain = np.full(shape, 2.0)
+    rn = np.full(shape, 5.0)
+    bg = np.zeros(shape)
+    o = _lib.default_options()
+    msg = ctypes.create_string_buffer(256)
+
+    def call(error, g, r, b):
+        img = _lib.CusmicImage(d.ctypes.data, None if error is None else error.ctypes.data,
+                               None if g is None else g.ctypes.data, None if r is None else r.ctypes.data,
+                               None if b is None else b.ctypes.data, None, w, h)
+        out, flags = np.full(shape, 7.0), np.full(shape, 7, np.uint8)
+        st = _lib.lib().remove_cosmics(ctypes.byref(img), ctypes.byref(o), out.ctypes.data,
+                                       flags.ctypes.data, msg, 256)
+        if st != _lib.OK:
+            assert (out == 7.0).all() and (flags == 7).all()
+        return st, msg.value.decode()
+
+    def bad(a, where, value):
+        a = a.copy()
+        a.flat[where] = value
+        return a
+
+    first, mid = 0, d.size // 2
+    say("map checks while uploading, error first", shape)
+    assert call(bad(err, first, 0.0), None, None, bg) == (_lib.INVALID, "error must be finite and positive")
+    say("map checks while uploading, background after a good error map", shape)
+    assert call(err, None, None, bad(bg, mid, math.inf)) == (_lib.INVALID, "background must be finite")
+    say("map checks while uploading, error before background", shape)
+    assert call(bad(err, mid, math.nan), None, None, bad(bg, first, math.inf)) == (
+        _lib.INVALID, "error must be finite and positive")
+    say("map checks while uploading, noise-model maps", shape)
+    assert call(None, bad(gain, mid, 0.0), rn, None) == (_lib.INVALID, "gain must be finite and positive")
+    assert call(None, gain, bad(rn, d.size - 1, -1.0), None) == (
+        _lib.INVALID, "read noise must be finite and nonnegative")
+    assert call(None, bad(gain, first, -1.0), bad(rn, first, -1.0), None) == (
+        _lib.INVALID, "gain must be finite and positive")
+    say("map checks while uploading, good maps", shape)
+    assert call(err, None, None, bg)[0] == _lib.OK
