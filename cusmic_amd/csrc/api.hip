@@ -823,6 +823,7 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	p.tile_skip = p.reseed_tag + nblocks;
 	p.changed_list = reinterpret_cast<unsigned long long *>(base + off_clist);
 	p.nblocks = nblocks;
+	p.list_cap = unsigned(planes);
 	p.dirty_list = reinterpret_cast<unsigned long long *>(base + off_dlist);
 	p.chunk_frame = reinterpret_cast<int *>(base + off_cframe);
 	p.run_frame = reinterpret_cast<int *>(base + off_rframe);
@@ -849,11 +850,21 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	p.reach = model ? REACH : REACH_MAPS;
 	p.reach_s = model ? REACH_S : REACH_S_MAPS;
 	timer tm(stream, profile != nullptr);
+	/* Without a background the output is C except at holes, and the
+	 * epilogue can touch only the flagged pixels and the holes, on a mask
+	 * plane zeroed here with the flags. Not when the output mask overlaps an
+	 * input mask, which the first pass has yet to read. */
+	bool sparse_end = !bg_any;
+	for (size_t f = 0; f < nf && sparse_end; ++f) {
+		const uintptr_t in = reinterpret_cast<uintptr_t>(ims[f].mask), out = reinterpret_cast<uintptr_t>(mask);
+		sparse_end = !in || in + ims[f].width * ims[f].height <= out || out + total <= in;
+	}
 	tm.mark(-1);
 	hipLaunchKernelGGL(zero_words, dim3(unsigned(std::min<size_t>((words + 255) / 256, 4096))),
 		dim3(256), 0, stream, w32, words);
 	hipLaunchKernelGGL(zero_groups, dim3(unsigned(std::min<size_t>((total / 16 + 256) / 256, size_t(cus) * 8))),
-		dim3(256), 0, stream, reinterpret_cast<uint4 *>(p.flags), (total + 15) / 16);
+		dim3(256), 0, stream, reinterpret_cast<uint4 *>(p.flags), (total + 15) / 16,
+		sparse_end ? mask : nullptr, total);
 	if (donors)
 		hipLaunchKernelGGL(reset_donors, dim3(unsigned(std::min<size_t>((2 * total + 255) / 256, 8192))),
 			dim3(256), 0, stream, donors, 2 * total);
@@ -950,7 +961,9 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 		if (!nl[nl_pass(b - 1, PF_ACTIVE)] || (*status_word & ERR_NO_DONORS))
 			break;
 	}
-	{
+	if (sparse_end) {
+		hipLaunchKernelGGL(epilogue_sparse, dim3(unsigned(cus) * 4), dim3(BLOCK), 0, stream, p);
+	} else {
 		int bg_maps = 0;
 		for (size_t f = 0; f < nf; ++f)
 			bg_maps |= ims[f].background != nullptr;

@@ -120,8 +120,8 @@ constexpr int PASS_RING = 2 * CHUNK_PASSES;
 
 /* Params::nlist: global counters, then PASS_FIELDS counters per ring slot. */
 enum {
-	NL_LIST,        /* length of the flagged-pixel list (after the hole list) */
-	NL_HOLES,       /* length of the hole list */
+	NL_LIST,        /* length of the flagged-pixel list (from the bottom of list) */
+	NL_HOLES,       /* length of the hole list (from the top of list) */
 	NL_WIDE_ALL,    /* length of wide_all */
 	NL_FILL_WIDE,   /* widened replacements of the hole fill */
 	NL_GLOBALS
@@ -188,6 +188,9 @@ struct Params {
 	int *run_frame;                  /* [nruns] frame of each first-pass run */
 	int nruns;
 	int nblocks;
+	unsigned list_cap;               /* entries that list holds: the flagged
+					  * pixels from the bottom, the holes (hole
+					  * list) from the top */
 	unsigned int *block_pass;        /* [nblocks] stamp of the last block pass
 					  * that processed the block, or 0 */
 	unsigned int *tile_skip;         /* [ntiles] nonzero if the first pass left

@@ -69,6 +69,23 @@ list_push(bool pred, T item, T *list, unsigned int *count)
 		list[base + __popcll(m & ((1ull << lane) - 1))] = item;
 }
 
+/* list_push into a list that fills from its top: entry k at end[-1 - k]. */
+template <class T>
+__device__ __forceinline__ void
+list_push_top(bool pred, T item, T *end, unsigned int *count)
+{
+	unsigned long long m = __ballot(pred);
+	if (!m)
+		return;
+	int lane = threadIdx.x & 63, leader = __ffsll((long long)m) - 1;
+	unsigned base = 0;
+	if (lane == leader)
+		base = atomicAdd(count, (unsigned)__popcll(m));
+	base = __shfl(base, leader, 64);
+	if (pred)
+		end[-1 - long(base + __popcll(m & ((1ull << lane) - 1)))] = item;
+}
+
 /* Per-workgroup output buffer of list entries: lanes append in LDS, then one
  * global atomic per flush reserves room in the shared list (a single counter
  * hammered by every wave serialises in L2). */
@@ -136,13 +153,27 @@ zero_words(unsigned int *words, size_t n)
 		words[i] = 0;
 }
 
-/* Zero n 16-byte groups (the flags plane). */
+/* Zero n 16-byte groups (the flags plane) and, when mask is given, the
+ * mask_bytes of the caller's mask plane, whose start may have any alignment:
+ * its bytes before the first 16-byte boundary and after the last one go one
+ * by one. */
 __global__ void
-zero_groups(uint4 *groups, size_t n)
+zero_groups(uint4 *groups, size_t n, uint8_t *mask, size_t mask_bytes)
 {
-	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-	     i += (size_t)gridDim.x * blockDim.x)
+	const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x, stride = (size_t)gridDim.x * blockDim.x;
+	for (size_t i = t; i < n; i += stride)
 		groups[i] = make_uint4(0, 0, 0, 0);
+	if (!mask)
+		return;
+	const size_t head = std::min<size_t>((16 - (reinterpret_cast<uintptr_t>(mask) & 15)) & 15, mask_bytes);
+	const size_t body = (mask_bytes - head) / 16, tail = head + body * 16;
+	uint4 *aligned = reinterpret_cast<uint4 *>(mask + head);
+	for (size_t i = t; i < body; i += stride)
+		aligned[i] = make_uint4(0, 0, 0, 0);
+	if (t < head)
+		mask[t] = 0;
+	if (t < mask_bytes - tail)
+		mask[tail + t] = 0;
 }
 
 /* Fold the pending per-frame count shards into the totals (and, after a
@@ -291,6 +322,27 @@ frame_from(const Frame *frames, int nf, int f, long long g)
 	while (f + 1 < nf && frames[f + 1].off <= g)
 		++f;
 	return f;
+}
+
+/* Epilogue without a background, where out = C except at holes: the mask
+ * plane was zeroed with the flags, and only the flagged pixels (the flagged
+ * list, from the bottom of list) and the holes (the hole list, from its top)
+ * are touched: mask 1 at the former, out = D (payload bits kept) at the
+ * latter. Holes are never flagged. The same output as epilogue. */
+__global__ __launch_bounds__(BLOCK) void
+epilogue_sparse(Params p)
+{
+	const unsigned flagged = p.nlist[NL_LIST], holes = p.nlist[NL_HOLES];
+	for (unsigned e = blockIdx.x * BLOCK + threadIdx.x; e < flagged + holes; e += gridDim.x * BLOCK) {
+		const bool hole = e >= flagged;
+		const unsigned long long entry = hole ? p.list[p.list_cap - 1 - (e - flagged)] : p.list[e];
+		const Frame &fr = p.frames[entry >> 32];
+		const long long i = (long long)(entry & 0xffffffffu);
+		if (hole)
+			p.clean[fr.off + i] = fr.data[i];
+		else
+			p.mask[fr.off + i] = 1;
+	}
 }
 
 /* Epilogue (src/image.cuh:50-70): out = finite D ? C - bg : D (payload bits
@@ -1043,7 +1095,7 @@ replace_pass(Params p, int iter, int fill)
 		unsigned long long entry = 0;
 		int f = 0, i = 0;
 		if (e < n) {
-			entry = e < n_new ? p.list[start + e] : p.list_b[e - n_new];
+			entry = fill ? p.list[p.list_cap - 1 - e] : e < n_new ? p.list[start + e] : p.list_b[e - n_new];
 			f = (int)(entry >> 32);
 			i = (int)(entry & 0xffffffffu);
 			if (replace_allowed(p, f, iter, fill)) {
@@ -2210,7 +2262,7 @@ significance_pass(SigShared<!MODEL> &sh, ModelRegion *region, const Params &p, i
 					if (hole)
 						p.flags[fr.off + i] = uint8_t(HOLE | EXCL);
 					acc.holes += hole;
-					list_push(hole, list_entry(f, i), p.list, &p.nlist[1]);
+					list_push_top(hole, list_entry(f, i), p.list + p.list_cap, &p.nlist[1]);
 				}
 			} else {
 				for (int j = 0; j < TILE_H / 4; ++j) {

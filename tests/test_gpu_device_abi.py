@@ -291,3 +291,51 @@ def test_host_entry_point_checks_maps_while_uploading(cuda, shape):
         _lib.INVALID, "gain must be finite and positive")
     say("map checks while uploading, good maps", shape)
     assert call(err, None, None, bg)[0] == _lib.OK
+
+
+@pytest.mark.parametrize("background", [False, True])
+def test_output_mask_in_place_and_unaligned(cuda, background):
+    """The output mask may be the input mask plane (it is read before the
+    output is written), and may start at any byte address: bytes just
+    outside it stay untouched. Both equal the oracle, with and without a
+    background (two ways of writing the outputs)."""
+    from cusmic_amd import _lib, scenes
+
+    h, w = 61, 203
+    data, signal = scenes.frame((h, w), 41, "cpu", n_stars=4, n_tracks=14, max_len=5)
+    host = data.numpy()
+    err = scenes.error_map(signal, 2.0, 5.0).numpy()
+    rng = np.random.default_rng(5)
+    in_mask = (rng.random((h, w)) < 0.02).astype(np.uint8)
+    bg = 7.25
+    o = _lib.default_options()
+    if background:
+        o.background = bg
+    want, wmask, _, _ = oracle.clean_frames([dict(data=host, error=err, mask=in_mask)],
+                                            background=bg if background else float("nan"))
+    assert wmask[0].sum() > 0
+    d, e = torch.from_numpy(host).to(cuda), torch.from_numpy(err).to(cuda)
+
+    def run(mask_in, mask_out):
+        im = _lib.CusmicImage(d.data_ptr(), e.data_ptr(), None, None, None, mask_in.data_ptr(), w, h)
+        clean = torch.empty(h * w, dtype=torch.float64, device=cuda)
+        st, msg = single_call(im, o, clean, mask_out, 0)
+        assert st == 0, msg
+        torch.cuda.synchronize()
+        return clean.cpu().numpy().reshape(h, w)
+
+    say("in place, background", background)
+    shared = torch.from_numpy(in_mask.ravel().copy()).to(cuda)
+    got = run(shared, shared)
+    assert_same_pixels(got, want[0])
+    np.testing.assert_array_equal(shared.cpu().numpy().reshape(h, w).astype(bool), wmask[0])
+
+    for shift in (1, 3, 8, 13):
+        say("output mask at byte offset", shift)
+        guard = torch.full((h * w + 32,), 0xAB, dtype=torch.uint8, device=cuda)
+        view = guard[shift:shift + h * w]
+        got = run(torch.from_numpy(in_mask.ravel().copy()).to(cuda), view)
+        assert_same_pixels(got, want[0])
+        g = guard.cpu().numpy()
+        np.testing.assert_array_equal(g[shift:shift + h * w].reshape(h, w).astype(bool), wmask[0])
+        assert (g[:shift] == 0xAB).all() and (g[shift + h * w:] == 0xAB).all()
