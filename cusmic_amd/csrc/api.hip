@@ -895,8 +895,8 @@ clean_images(const cusmic_image *ims, size_t nf, const cusmic_options &o, const 
 			hipLaunchKernelGGL(expand_dirty, dim3(unsigned(cus) * 2), dim3(BLOCK), 0, stream, p, it);
 		hipLaunchKernelGGL(blocks, dim3(blocks_grid), dim3(BLOCK), 0, stream, p, it);
 		tm.mark(CUSMIC_PROFILE_SIGNIFICANCE_MS);
-		hipLaunchKernelGGL(fine<true>, dim3(list_grid), dim3(BLOCK), 0, stream, p, it);
-		hipLaunchKernelGGL(fine<false>, dim3(list_grid), dim3(BLOCK), 0, stream, p, it);
+		hipLaunchKernelGGL(fine, dim3(list_grid), dim3(BLOCK), 0, stream, p, it);
+		hipLaunchKernelGGL(fine_edge, dim3(list_grid), dim3(BLOCK), 0, stream, p, it);
 		hipLaunchKernelGGL(grow_first, dim3(list_grid), dim3(BLOCK), 0, stream, p, it);
 		hipLaunchKernelGGL(grow_second, dim3(list_grid), dim3(BLOCK), 0, stream, p, it);
 		/* the pass's counts; after the first pass redo_scan folds them */

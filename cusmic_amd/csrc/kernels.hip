@@ -1224,8 +1224,9 @@ redo_scan(Params p, int iter)
  *                 pixel the last replacement changed.
  *   fine          per seed: F = max((m3 - med7(m3)) / N, 0.01), m3 = med3(C);
  *                 S' / F > contrast puts the seed on the candidate list (two
- *                 launches: seeds whose 9 x 9 samples lie inside the frame,
- *                 then those near an edge)
+ *                 launches: fine, one lane per seed whose 9 x 9 samples lie
+ *                 inside the frame, then fine_edge, one wave per seed near an
+ *                 edge)
  *   grow_first    per candidate: neighbours with S' > t join the growth list
  *   grow_second   per grown pixel: neighbours with S' > t_n are flagged; a
  *                 pixel flagged for the first time is new (accumulate)
@@ -2748,117 +2749,58 @@ fine_inner(int x, int y, int w, int h)
 	return x >= 4 && x + 4 < w && y >= 4 && y + 4 < h;
 }
 
-/* F at a seed (src/filters.cuh:148-162): m3 = med3 of the clean image (median
- * border mode) over the 7 x 7 positions around it, med7 = their median, fed row
- * by row so that at most 22 ranks stay live, and N at the seed. INNER: the seed
- * is fine_inner, and the code has no border paths (about 220 VGPRs, two waves
- * per SIMD, where the border paths need more than 256). */
-template <bool INNER>
+/* F at a seed whose 9 x 9 samples lie inside the frame (fine_inner;
+ * src/filters.cuh:148-162): m3 = med3 of the clean image over the 7 x 7
+ * positions around it, med7 = their median, fed row by row so that at most 22
+ * ranks stay live, and N at the seed. Rows are loaded one at a time through
+ * one row pointer (about 220 VGPRs, two waves per SIMD). fine_edge takes the
+ * seeds near an edge. */
 __device__ double
 fine_structure(const Params &p, const Frame &fr, int x, int y)
 {
 	const double *clean = p.clean + fr.off;
-	const int w = fr.w, h = fr.h, mode = p.mode;
-	const bool inside = INNER || (x >= 3 && x + 3 < w && y >= 3 && y + 3 < h);
-	/* in the extended image of a symmetric or periodic mode, med3 at an outside
-	 * position equals med3 at its mapped position; CONSTANT positions outside
-	 * the frame are 0; NEAREST near the edge takes the clamped position */
-	const bool block = inside || mode != CUSMIC_NEAREST;
-	/* the 9 x 9 block of C samples: INNER reads it through one row pointer;
-	 * otherwise its columns and each row are mapped once (border_index, -1 for
-	 * a CONSTANT sample outside, which is 0) */
-	int bx[9];
-#pragma unroll
-	for (int i = 0; i < 9; ++i)
-		bx[i] = INNER ? x - 4 + i : border_index(x - 4 + i, w, mode);
+	const int w = fr.w;
 	auto load_row = [&](int yy, double *dst) {
-		if (INNER) {
-			const double *r = clean + (long long)yy * w;
+		const double *r = clean + (long long)yy * w + x - 4;
 #pragma unroll
-			for (int i = 0; i < 9; ++i)
-				dst[i] = r[bx[i]];
-		} else {
-			const int by = border_index(yy, h, mode);
-			const double *r = clean + (long long)max(by, 0) * w;
-#pragma unroll
-			for (int i = 0; i < 9; ++i) {
-				const double v = r[max(bx[i], 0)];
-				dst[i] = by < 0 || bx[i] < 0 ? 0.0 : v;
-			}
-		}
+		for (int i = 0; i < 9; ++i)
+			dst[i] = r[i];
 	};
 	double acc[2][22], a[9], b[9], c[9], centre = 0.0;
-	/* the edge launch (one wave per SIMD) loads the whole block at once,
-	 * the inner one row by row */
-	double all[INNER ? 1 : 9][9];
-	if (block) {
-		if constexpr (INNER) {
-			load_row(y - 4, a);
-			load_row(y - 3, b);
-		} else {
-#pragma unroll
-			for (int r = 0; r < 9; ++r)
-				load_row(y - 4 + r, all[r]);
-#pragma unroll
-			for (int i = 0; i < 9; ++i) {
-				a[i] = all[0][i];
-				b[i] = all[1][i];
-			}
-		}
-	}
+	load_row(y - 4, a);
+	load_row(y - 3, b);
 #pragma unroll
 	for (int j = 0; j < 7; ++j) {
 		double row[7];
-		if (block) {
-			if constexpr (INNER) {
-				load_row(y - 2 + j, c);
-			} else {
+		load_row(y - 2 + j, c);
+		/* 3x3 medians from sorted columns: med9 = med3(max of minima, med3
+		 * of middles, min of maxima) */
+		double lo[9], md[9], hi[9];
 #pragma unroll
-				for (int i = 0; i < 9; ++i)
-					c[i] = all[j + 2][i];
-			}
-			/* 3x3 medians from sorted columns: med9 = med3(max of minima,
-			 * med3 of middles, min of maxima) */
-			double lo[9], md[9], hi[9];
+		for (int i = 0; i < 9; ++i) {
+			double l1 = fmin(a[i], b[i]), h1 = fmax(a[i], b[i]);
+			double t = fmax(l1, c[i]);
+			lo[i] = fmin(l1, c[i]);
+			md[i] = fmin(h1, t);
+			hi[i] = fmax(h1, t);
+		}
 #pragma unroll
-			for (int i = 0; i < 9; ++i) {
-				double l1 = fmin(a[i], b[i]), h1 = fmax(a[i], b[i]);
-				double t = fmax(l1, c[i]);
-				lo[i] = fmin(l1, c[i]);
-				md[i] = fmin(h1, t);
-				hi[i] = fmax(h1, t);
-			}
+		for (int i = 0; i < 7; ++i) {
+			double L = fmax(fmax(lo[i], lo[i + 1]), lo[i + 2]);
+			double H = fmin(fmin(hi[i], hi[i + 1]), hi[i + 2]);
+			double M = med3(md[i], md[i + 1], md[i + 2]);
+			row[i] = med3(L, M, H);
+		}
 #pragma unroll
-			for (int i = 0; i < 7; ++i) {
-				double L = fmax(fmax(lo[i], lo[i + 1]), lo[i + 2]);
-				double H = fmin(fmin(hi[i], hi[i + 1]), hi[i + 2]);
-				double M = med3(md[i], md[i + 1], md[i + 2]);
-				double m = med3(L, M, H);
-				int qx = x - 3 + i, qy = y - 3 + j;
-				bool out = qx < 0 || qx >= w || qy < 0 || qy >= h;
-				row[i] = mode == CUSMIC_CONSTANT && out ? 0.0 : m;
-			}
-#pragma unroll
-			for (int i = 0; i < 9; ++i) {
-				a[i] = b[i];
-				b[i] = c[i];
-			}
-		} else {
-			for (int i = 0; i < 7; ++i) {
-				int qx = min(max(x - 3 + i, 0), w - 1), qy = min(max(y - 3 + j, 0), h - 1);
-				double v9[9];
-				for (int r = 0; r < 3; ++r)
-					for (int q = 0; q < 3; ++q)
-						v9[r * 3 + q] = med_sample(clean, qx - 1 + q, qy - 1 + r, w, h, mode);
-				median9(v9, &row[i]);
-			}
+		for (int i = 0; i < 9; ++i) {
+			a[i] = b[i];
+			b[i] = c[i];
 		}
 		if (j == 3)
 			centre = row[3];
 		median49_feed(j, acc[(j + 1) & 1], row, acc[j & 1]);
 		/* keep the loads of later rows from being hoisted here */
-		if (INNER)
-			__builtin_amdgcn_sched_barrier(0);
+		__builtin_amdgcn_sched_barrier(0);
 	}
 	const double med7 = acc[0][0];
 	double nz;
@@ -2881,16 +2823,15 @@ fine_structure(const Params &p, const Frame &fr, int x, int y)
 	return fine < FINE_FLOOR ? FINE_FLOOR : fine;
 }
 
-/* Candidates among the seeds of pass iter: S' / F > contrast. The INNER
- * launch reads the seeds, takes the fine_inner ones and lists the others not
- * excluded (edge_list); the second launch takes those. */
-template <bool INNER>
+/* Candidates among the seeds of pass iter: S' / F > contrast. This launch
+ * takes the fine_inner seeds and lists the others not excluded (edge_list),
+ * which fine_edge takes. */
 __global__ __launch_bounds__(BLOCK) void
 fine(Params p, int iter)
 {
 	__shared__ BlockList<2 * BLOCK> out; /* flushed when the next round might not fit */
-	const unsigned n = p.nlist[INNER ? nl_seed(p, iter) : nl_edge(p, iter)];
-	const unsigned long long *seeds = INNER ? p.list_b : p.edge_list;
+	const unsigned n = p.nlist[nl_seed(p, iter)];
+	const unsigned long long *seeds = p.list_b;
 	unsigned int *count = &p.nlist[nl_cand(p, iter)];
 	if (!threadIdx.x)
 		out.n = 0;
@@ -2904,18 +2845,125 @@ fine(Params p, int iter)
 			const Frame fr = p.frames[f];
 			const int x = i % fr.w, y = i / fr.w;
 			if (!(p.flags[fr.off + i] & EXCL)) {
-				if (INNER && !fine_inner(x, y, fr.w, fr.h)) {
+				if (!fine_inner(x, y, fr.w, fr.h)) {
 					edge = true;
 				} else {
 					double sp = p.sprime[fr.off + i];
-					cand = sp / fine_structure<INNER>(p, fr, x, y) > p.contrast;
+					cand = sp / fine_structure(p, fr, x, y) > p.contrast;
 				}
 			}
 		}
-		if (INNER)
-			list_push(edge, entry, p.edge_list, &p.nlist[nl_edge(p, iter)]);
+		list_push(edge, entry, p.edge_list, &p.nlist[nl_edge(p, iter)]);
 		__syncthreads();
 		lds_append(cand, entry, out.item, &out.n);
+		__syncthreads();
+		if (out.n > BLOCK)
+			block_list_flush(out, p.list_c, count);
+	}
+	block_list_flush(out, p.list_c, count);
+}
+
+/* Rank of lane k's value among the first n of vals (ties by lane): the
+ * values' ranks are 0 .. n - 1, each once, so the lane of rank n / 2 holds
+ * the value that a median network over them returns. */
+__device__ __forceinline__ int
+lane_rank(const double *vals, int n, int k)
+{
+	const double v = vals[k];
+	int r = 0;
+	for (int j = 0; j < n; ++j)
+		r += vals[j] < v || (vals[j] == v && j < k);
+	return r;
+}
+
+/* The seeds that fine left on the edge list (their 9 x 9 samples cross
+ * the frame edge): F as in fine_structure, one wave per seed. Lane k < 49
+ * takes m3 at position k of the 7 x 7 window under the same border rules
+ * (med3 of mapped samples; CONSTANT m3 outside the frame 0; NEAREST off the
+ * inner ring at the clamped position), and the wave selects med7 by rank,
+ * as does the noise model's 5 x 5 median. C holds no nonfinite value by now,
+ * and a signed zero among the medians does not change F (centre - med7 and
+ * the noise floor), so F equals fine_structure's. */
+__global__ __launch_bounds__(BLOCK) void
+fine_edge(Params p, int iter)
+{
+	__shared__ BlockList<2 * BLOCK> out; /* flushed when the next round might not fit */
+	__shared__ double vals[BLOCK / 64][64];
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const unsigned n = p.nlist[nl_edge(p, iter)];
+	unsigned int *count = &p.nlist[nl_cand(p, iter)];
+	const int mode = p.mode;
+	if (!threadIdx.x)
+		out.n = 0;
+	double *v = vals[wave];
+	for (unsigned base = blockIdx.x * (BLOCK / 64); base < n; base += gridDim.x * (BLOCK / 64)) {
+		const unsigned e = base + wave;
+		bool cand = false;
+		unsigned long long entry = 0;
+		if (e < n) {
+			entry = uniform_u64(p.edge_list[e]);
+			const int f = (int)(entry >> 32), i = (int)(entry & 0xffffffffu);
+			const Frame fr = p.frames[f];
+			const int w = fr.w, h = fr.h, x = i % w, y = i / w;
+			const double *clean = p.clean + fr.off;
+			const bool inside = x >= 3 && x + 3 < w && y >= 3 && y + 3 < h;
+			double m = 0.0;
+			if (lane < 49) {
+				const int qx = x - 3 + lane % 7, qy = y - 3 + lane / 7;
+				double v9[9];
+				if (inside || mode != CUSMIC_NEAREST) {
+#pragma unroll
+					for (int k = 0; k < 9; ++k)
+						v9[k] = med_sample(clean, qx - 1 + k % 3, qy - 1 + k / 3, w, h, mode);
+					median9(v9, &m);
+					const bool outside = qx < 0 || qx >= w || qy < 0 || qy >= h;
+					if (mode == CUSMIC_CONSTANT && outside)
+						m = 0.0;
+				} else {
+					const int cx = min(max(qx, 0), w - 1), cy = min(max(qy, 0), h - 1);
+#pragma unroll
+					for (int k = 0; k < 9; ++k)
+						v9[k] = med_sample(clean, cx - 1 + k % 3, cy - 1 + k / 3, w, h, mode);
+					median9(v9, &m);
+				}
+				v[lane] = m;
+			}
+			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+			__builtin_amdgcn_wave_barrier();
+			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+			const double centre = v[24];
+			const unsigned long long mid = __ballot(lane < 49 && lane_rank(v, 49, lane) == 24);
+			const double med7 = __shfl(m, __ffsll((long long)mid) - 1, 64);
+			double nz;
+			if (fr.error) {
+				nz = fr.error[i];
+			} else {
+				__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+				__builtin_amdgcn_wave_barrier();
+				__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+				double c = 0.0;
+				if (lane < 25) {
+					c = med_sample(clean, x - 2 + lane % 5, y - 2 + lane / 5, w, h, mode);
+					v[lane] = c;
+				}
+				__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+				__builtin_amdgcn_wave_barrier();
+				__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+				const unsigned long long m5l = __ballot(lane < 25 && lane_rank(v, 25, lane) == 12);
+				const double m5 = __shfl(c, __ffsll((long long)m5l) - 1, 64);
+				const double g = fr.gain ? fr.gain[i] : fr.g;
+				const double rn = fr.readnoise ? fr.readnoise[i] : fr.rn;
+				nz = noise_model(m5, g, rn);
+			}
+			double fine = (centre - med7) / nz;
+			fine = fine < FINE_FLOOR ? FINE_FLOOR : fine;
+			cand = !(p.flags[fr.off + i] & EXCL) && p.sprime[fr.off + i] / fine > p.contrast;
+			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+			__builtin_amdgcn_wave_barrier();
+			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+		}
+		__syncthreads();
+		lds_append(cand && !lane, entry, out.item, &out.n);
 		__syncthreads();
 		if (out.n > BLOCK)
 			block_list_flush(out, p.list_c, count);
